@@ -1,0 +1,132 @@
+/*
+ * mds_hip.h -- C ABI of libmds_hip.so, the MI355X (gfx950) engine for the
+ * MassiveMDS pairwise (left-truncated) normal MDS log-likelihood and its
+ * gradient with respect to the N x P latent locations.
+ *
+ * Each entry point below replaces one virtual of the reference's engine
+ * interface, mds::AbstractMultiDimensionalScaling
+ * (src/AbstractMultiDimensionalScaling.hpp:47-59), or the factory
+ * (src/factory.cpp:32-118), with the same argument meaning:
+ *   - all double* arrays are caller-owned host buffers, copied in or out
+ *     before the call returns (src/NewMultiDimensionalScaling.hpp:160-163,
+ *     259, 301);
+ *   - locations and gradients are row-major N x P; observations are the
+ *     N x N matrix, row-major, NaN = missing, diagonal ignored.
+ * Every function returns 0 on success and a non-zero status otherwise (the
+ * reference only asserts); mds_hip_last_error() gives the message.
+ * Instances are independent and not thread-safe, as in the reference.
+ */
+#ifndef MDS_HIP_H
+#define MDS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Flag bits, identical to the reference's mds::Flags (src/flags.h:6-14). */
+#define MDS_HIP_FLAG_FLOAT (1L << 2)           /* FP32 engine (compensated sums) */
+#define MDS_HIP_FLAG_TBB (1L << 3)             /* accepted, ignored by the GPU engine */
+#define MDS_HIP_FLAG_OPENCL (1L << 4)          /* "GPU engine": routed to HIP */
+#define MDS_HIP_FLAG_LEFT_TRUNCATION (1L << 5)
+#define MDS_HIP_FLAG_SSE (1L << 7)             /* accepted, ignored */
+#define MDS_HIP_FLAG_AVX (1L << 8)             /* accepted, ignored */
+#define MDS_HIP_FLAG_AVX512 (1L << 9)          /* accepted, ignored */
+#define MDS_HIP_FLAG_HIP (1L << 10)            /* explicit alias of the GPU route */
+
+/* Status codes. */
+#define MDS_HIP_OK 0
+#define MDS_HIP_ERR_ARGUMENT 1   /* bad length, dimension, index or parameter */
+#define MDS_HIP_ERR_DEVICE 2     /* HIP runtime error or no GPU */
+#define MDS_HIP_ERR_STATE 3      /* call made before the data it needs were set */
+
+typedef struct mds_hip_engine* mds_hip_handle;
+
+/* Largest embedding dimension P the kernels accept (the OpenCL engine it
+ * replaces stops at 8, src/backend/opencl/instantiate.cpp:16-31). */
+int mds_hip_max_dimension(void);
+
+/* Number of visible HIP devices (0 when none). */
+int mds_hip_device_count(int* count);
+
+/*
+ * Construction, replacing constructOpenCLMultiDimensionalScalingDouble/Float
+ * (src/backend/opencl/instantiate.cpp:14-52) as reached from
+ * mds::factory(P, N, flags, device, threads) (src/factory.cpp:32-118).
+ * `device` is an index into the HIP devices; out of range selects device 0,
+ * as the OpenCL engine falls back to its default device
+ * (src/backend/opencl/OpenCLMultiDimensionalScaling.hpp:85-89).
+ */
+int mds_hip_create(int embeddingDimension, int locationCount, long flags, int device,
+                   mds_hip_handle* out);
+int mds_hip_destroy(mds_hip_handle h);
+
+/* Message of the last failed call on `h` (or of the last failed create when
+ * h is NULL); never NULL. */
+const char* mds_hip_last_error(mds_hip_handle h);
+
+/* virtual void updateLocations(int, double*, size_t)
+ * (src/AbstractMultiDimensionalScaling.hpp:48; semantics
+ * src/NewMultiDimensionalScaling.hpp:134-166).  locationIndex = -1 updates
+ * all N rows (length N*P); locationIndex >= 0 updates one row (length P). */
+int mds_hip_update_locations(mds_hip_handle h, int locationIndex, const double* location,
+                             size_t length);
+
+/* virtual double getSumOfIncrements() (:51; src/NewMultiDimensionalScaling.hpp:193-203). */
+int mds_hip_get_sum_of_increments(mds_hip_handle h, double* out);
+
+/* virtual void getLogLikelihoodGradient(double*, size_t) (:52;
+ * src/NewMultiDimensionalScaling.hpp:281-302).  length = N*P. */
+int mds_hip_get_log_likelihood_gradient(mds_hip_handle h, double* result, size_t length);
+
+/* virtual void storeState() / restoreState() / acceptState() (:53-55;
+ * src/NewMultiDimensionalScaling.hpp:205-255). */
+int mds_hip_store_state(mds_hip_handle h);
+int mds_hip_restore_state(mds_hip_handle h);
+int mds_hip_accept_state(mds_hip_handle h);
+
+/* virtual void setPairwiseData(double*, size_t) (:56;
+ * src/NewMultiDimensionalScaling.hpp:257-260).  length = N*N. */
+int mds_hip_set_pairwise_data(mds_hip_handle h, const double* data, size_t length);
+
+/* virtual void setParameters(double*, size_t) (:57;
+ * src/NewMultiDimensionalScaling.hpp:262-274).  length = 1: the precision
+ * tau, which must be finite and > 0. */
+int mds_hip_set_parameters(mds_hip_handle h, const double* data, size_t length);
+
+/* virtual void makeDirty() (:58; src/NewMultiDimensionalScaling.hpp:276-279). */
+int mds_hip_make_dirty(mds_hip_handle h);
+
+/* virtual int getInternalDimension() (:59); returns P. */
+int mds_hip_get_internal_dimension(mds_hip_handle h, int* out);
+
+/*
+ * Device-resident extensions (no counterpart in the reference; used by
+ * samplers and benchmarks that keep the state on the GPU).
+ *
+ * mds_hip_evaluate_async enqueues one evaluation at the current state on the
+ * instance's stream (likelihood, plus the gradient when computeGradient is
+ * non-zero) without any host copy; results stay on the device and also fill
+ * the engine's caches when read through the calls above.
+ */
+int mds_hip_evaluate_async(mds_hip_handle h, int computeGradient);
+int mds_hip_synchronize(mds_hip_handle h);
+
+/* Kernel timing with HIP events on the instance stream.  While enabled,
+ * every evaluation records the tile-kernel and whole-evaluation durations;
+ * get_timing synchronises and returns their sums (ms) and the count. */
+int mds_hip_set_timing(mds_hip_handle h, int enable);
+int mds_hip_get_timing(mds_hip_handle h, double* tileMs, double* evalMs, long long* count);
+
+/* Layout facts: 1 when the stored observations are symmetric (one tile
+ * plane), bytes of packed observations on the device, number of tiles. */
+int mds_hip_get_layout(mds_hip_handle h, int* symmetric, long long* observationBytes,
+                       long long* tiles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDS_HIP_H */

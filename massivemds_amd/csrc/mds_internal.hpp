@@ -1,0 +1,61 @@
+// Host-side declarations shared by the kernel launchers and the engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdship {
+
+constexpr int kWave = 64;                 // CDNA wavefront width
+constexpr int kTile = 64;                 // tile edge: 64 rows (one per lane) x 64 columns
+constexpr int kTileElems = kTile * kTile;
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlockThreads = kWave * kWavesPerBlock;
+
+// Kernel embedding dimensions that are compiled; a location dimension P runs
+// on the smallest of these >= P with zero-padded coordinates (padding adds 0
+// to every distance and gradient).
+constexpr int kKernelDims[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16};
+constexpr int kMaxDim = 16;
+
+inline int kernel_dim(int p) {
+    for (int d : kKernelDims)
+        if (d >= p) return d;
+    return -1;
+}
+
+// One fused evaluation over the packed upper-triangle tiles.
+struct TileLaunch {
+    const void* delta;       // packed tiles, Real, [planes][numTiles][64 cols][64 rows]
+    const void* x;           // Real [nb * 64][kdim], zero padded
+    const uint32_t* tiles;   // (I << 16) | J for I <= J, one per tile
+    int numTiles;
+    int nb;                  // 64-row blocks
+    int kdim;                // kernel dimension
+    int64_t planeStride;     // elements between the two planes (asymmetric data)
+    double tau;
+    bool truncated;
+    bool symmetric;
+    bool gradient;
+    double* tileSums;        // [numTiles]
+    double* part;            // [nb][nb][kdim][64] gradient partials
+};
+
+struct ReduceLaunch {
+    const double* part;
+    const double* tileSums;
+    int numTiles;
+    int nb;
+    int kdim;
+    int n;
+    int p;                   // output dimension (<= kdim)
+    bool gradient;
+    double* grad;            // [n][p]
+    double* sumOut;          // 1 double: 0.5 * sum of tile sums
+};
+
+hipError_t launch_tiles_f64(const TileLaunch& a, hipStream_t s);
+hipError_t launch_tiles_f32(const TileLaunch& a, hipStream_t s);
+hipError_t launch_reduce(const ReduceLaunch& a, hipStream_t s);
+
+}  // namespace mdship

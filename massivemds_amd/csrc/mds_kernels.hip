@@ -1,0 +1,245 @@
+// gfx950 kernels for the pairwise truncated-normal MDS log-likelihood and its
+// gradient.
+//
+// Work decomposition.  The N x N observation matrix is stored by the engine
+// as packed 64 x 64 tiles of its upper block triangle (tile (I, J), I <= J),
+// each tile column-major so that one wavefront reads one tile column (64
+// consecutive values) per load.  Symmetric data keep one plane; asymmetric
+// data keep a second plane with the transposed tile (delta_ji), so the
+// reference's row-wise semantics (row i uses delta_ij, row j uses delta_ji;
+// src/NewMultiDimensionalScaling.hpp:421-461, 494-522) are reproduced while
+// every transcendental is evaluated once per unordered pair.
+//
+// One wavefront owns one tile: lane l is row i = 64 I + l and loops over the
+// 64 columns j.  Row partials of the gradient stay in registers; column
+// partials are reduced across the wavefront with DPP and kept by lane j.
+// Each tile writes its likelihood partial and its two 64-row gradient
+// partials to fixed slots; a second kernel adds them in a fixed order in
+// FP64.  No atomics anywhere, so results are bitwise reproducible.
+#include "mds_device.hpp"
+#include "mds_internal.hpp"
+
+namespace mdship {
+
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
+__global__ __launch_bounds__(kBlockThreads) void tile_kernel(
+    const Real* __restrict__ delta, const Real* __restrict__ x,
+    const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
+    Real tau, Real halfTau, Real sqrtTau, double* __restrict__ tileSums,
+    double* __restrict__ part) {
+    const int lane = threadIdx.x % kWave;
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
+    if (t >= numTiles) return;
+    const uint32_t code = tiles[t];
+    const int I = (int)(code >> 16);
+    const int J = (int)(code & 0xffffu);
+    const bool diag = (I == J);
+
+    Real xi[P];
+    const Real* xrow = x + ((int64_t)I * kTile + lane) * P;
+#pragma unroll
+    for (int p = 0; p < P; ++p) xi[p] = xrow[p];
+
+    LaneSum<Real> sAcc;
+    LaneSum<Real> rowAcc[P];
+    Real colAcc[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) colAcc[p] = Real(0);
+
+    const Real* tileCol = delta + (int64_t)t * kTileElems + lane;
+    const Real* xcol = x + (int64_t)J * kTile * P;
+
+    for (int jl = 0; jl < kTile; ++jl) {
+        Real xj[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) xj[p] = xcol[jl * P + p];
+        const Real dij = tileCol[jl * kTile];
+        const Real dji = SYM ? dij : tileCol[planeStride + jl * kTile];
+        const bool inTri = !diag || lane < jl;
+        const bool vij = inTri && !m_isnan(dij);
+        const bool vji = inTri && !m_isnan(dji);
+
+        Real diff[P];
+        Real d2 = Real(0);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            diff[p] = xi[p] - xj[p];
+            d2 += diff[p] * diff[p];
+        }
+        const Real d = m_sqrt(d2);
+        const Real rij = dij - d;
+        const Real rji = dji - d;
+
+        Real lij, lji;
+        Real gt = Real(0);
+        if (TRUNC) {
+            const TruncTerms<Real> tt = trunc_terms<Real>(d * sqrtTau, sqrtTau);
+            lij = halfTau * rij * rij + tt.logPhi;
+            lji = halfTau * rji * rji + tt.logPhi;
+            gt = tt.gradTerm;
+        } else {
+            lij = rij * rij;
+            lji = rji * rji;
+        }
+        if (SYM)
+            sAcc.add(vij ? lij + lij : Real(0));
+        else
+            sAcc.add((vij ? lij : Real(0)) + (vji ? lji : Real(0)));
+
+        if (GRAD) {
+            // c = (r - phi / (Phi sqrt tau)) * tau / d for each ordered pair
+            // (src/NewMultiDimensionalScaling.hpp:435-444); masked pairs give 0.
+            const Real tauOverD = tau / d;
+            const Real cij = vij ? (rij - gt) * tauOverD : Real(0);
+            const Real cji = SYM ? cij : (vji ? (rji - gt) * tauOverD : Real(0));
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const Real rowTerm = cij * diff[p];
+                rowAcc[p].add(rowTerm);
+                const Real colTerm = SYM ? -rowTerm : -(cji * diff[p]);
+                const Real col = read_last_lane(wave_sum_to_last(colTerm));
+                colAcc[p] = (lane == jl) ? col : colAcc[p];
+            }
+        }
+    }
+
+    const double s = wave_sum_to_last(sAcc.value());
+    if (lane == kWave - 1) tileSums[t] = s;
+
+    if (GRAD) {
+        double* rowSlot = part + ((int64_t)I * nb + J) * (P * kTile) + lane;
+        if (diag) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) rowSlot[p * kTile] = rowAcc[p].value() + (double)colAcc[p];
+        } else {
+            double* colSlot = part + ((int64_t)J * nb + I) * (P * kTile) + lane;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                rowSlot[p * kTile] = rowAcc[p].value();
+                colSlot[p * kTile] = (double)colAcc[p];
+            }
+        }
+    }
+}
+
+constexpr int kReduceWaves = 8;
+constexpr int kReduceThreads = kReduceWaves * kWave;
+
+// Block b < nb: gradient rows of row block b, 8 wavefronts splitting the
+// column blocks J (J = w mod 8), combined through LDS in fixed order.
+// Last block: the likelihood, a fixed-order FP64 sum of the tile partials,
+// halved as in src/NewMultiDimensionalScaling.hpp:556.
+template <int P>
+__global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
+    const double* __restrict__ part, const double* __restrict__ tileSums, int numTiles,
+    int nb, int n, int pOut, double* __restrict__ grad, double* __restrict__ sumOut) {
+    __shared__ double red[kReduceThreads * (P > 1 ? P : 1)];
+    const int tid = threadIdx.x;
+    if (blockIdx.x == gridDim.x - 1) {
+        double acc = 0.0;
+        for (int k = tid; k < numTiles; k += kReduceThreads) acc += tileSums[k];
+        red[tid] = acc;
+        __syncthreads();
+        for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
+            if (tid < s) red[tid] += red[tid + s];
+            __syncthreads();
+        }
+        if (tid == 0) *sumOut = 0.5 * red[0];
+        return;
+    }
+    const int K = blockIdx.x;
+    const int wave = tid / kWave;
+    const int lane = tid % kWave;
+    double acc[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) acc[p] = 0.0;
+    const double* base = part + (int64_t)K * nb * (P * kTile) + lane;
+    for (int J = wave; J < nb; J += kReduceWaves) {
+        const double* slot = base + (int64_t)J * (P * kTile);
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] += slot[p * kTile];
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) red[(p * kReduceWaves + wave) * kWave + lane] = acc[p];
+    __syncthreads();
+    if (wave == 0) {
+        const int row = K * kTile + lane;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            double v = 0.0;
+#pragma unroll
+            for (int w = 0; w < kReduceWaves; ++w) v += red[(p * kReduceWaves + w) * kWave + lane];
+            if (row < n && p < pOut) grad[(int64_t)row * pOut + p] = v;
+        }
+    }
+}
+
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
+static hipError_t go(const TileLaunch& a, hipStream_t s) {
+    const dim3 grid((a.numTiles + kWavesPerBlock - 1) / kWavesPerBlock);
+    const Real tau = (Real)a.tau;
+    const Real halfTau = (Real)(0.5 * a.tau);
+    const Real sqrtTau = (Real)sqrt(a.tau);
+    hipLaunchKernelGGL((tile_kernel<Real, P, TRUNC, SYM, GRAD>), grid, dim3(kBlockThreads), 0, s,
+                       (const Real*)a.delta, (const Real*)a.x, a.tiles, a.numTiles, a.nb,
+                       a.planeStride, tau, halfTau, sqrtTau, a.tileSums, a.part);
+    return hipGetLastError();
+}
+
+template <typename Real, int P>
+static hipError_t dispatch_flags(const TileLaunch& a, hipStream_t s) {
+    if (a.truncated) {
+        if (a.symmetric)
+            return a.gradient ? go<Real, P, true, true, true>(a, s) : go<Real, P, true, true, false>(a, s);
+        return a.gradient ? go<Real, P, true, false, true>(a, s) : go<Real, P, true, false, false>(a, s);
+    }
+    if (a.symmetric)
+        return a.gradient ? go<Real, P, false, true, true>(a, s) : go<Real, P, false, true, false>(a, s);
+    return a.gradient ? go<Real, P, false, false, true>(a, s) : go<Real, P, false, false, false>(a, s);
+}
+
+template <typename Real>
+static hipError_t dispatch_dim(const TileLaunch& a, hipStream_t s) {
+    switch (a.kdim) {
+        case 1: return dispatch_flags<Real, 1>(a, s);
+        case 2: return dispatch_flags<Real, 2>(a, s);
+        case 3: return dispatch_flags<Real, 3>(a, s);
+        case 4: return dispatch_flags<Real, 4>(a, s);
+        case 5: return dispatch_flags<Real, 5>(a, s);
+        case 6: return dispatch_flags<Real, 6>(a, s);
+        case 8: return dispatch_flags<Real, 8>(a, s);
+        case 10: return dispatch_flags<Real, 10>(a, s);
+        case 12: return dispatch_flags<Real, 12>(a, s);
+        case 16: return dispatch_flags<Real, 16>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_tiles_f64(const TileLaunch& a, hipStream_t s) { return dispatch_dim<double>(a, s); }
+hipError_t launch_tiles_f32(const TileLaunch& a, hipStream_t s) { return dispatch_dim<float>(a, s); }
+
+template <int P>
+static hipError_t reduce_p(const ReduceLaunch& a, hipStream_t s) {
+    const dim3 grid(a.gradient ? a.nb + 1 : 1);
+    hipLaunchKernelGGL((reduce_kernel<P>), grid, dim3(kReduceThreads), 0, s, a.part, a.tileSums,
+                       a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const ReduceLaunch& a, hipStream_t s) {
+    switch (a.kdim) {
+        case 1: return reduce_p<1>(a, s);
+        case 2: return reduce_p<2>(a, s);
+        case 3: return reduce_p<3>(a, s);
+        case 4: return reduce_p<4>(a, s);
+        case 5: return reduce_p<5>(a, s);
+        case 6: return reduce_p<6>(a, s);
+        case 8: return reduce_p<8>(a, s);
+        case 10: return reduce_p<10>(a, s);
+        case 12: return reduce_p<12>(a, s);
+        case 16: return reduce_p<16>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace mdship

@@ -1,0 +1,128 @@
+"""Portable synthetic inputs for the MDS likelihood.
+
+The reference benchmark draws its data with ``std::mt19937`` and
+``std::normal_distribution`` (src/benchmark.cpp:63-81, 174-189), whose output is
+implementation-defined.  These generators are instead fully specified so that
+the same seed gives the same inputs in any process that uses this module:
+
+* uniforms: a splitmix64 stream per (seed, key); each 64-bit output ``u``
+  maps to ``((u >> 11) + 0.5) * 2**-53`` in (0, 1);
+* normals: Box-Muller on consecutive uniform pairs (``r = sqrt(-2 log u1)``,
+  ``theta = 2 pi u2``, giving ``r cos theta`` then ``r sin theta``).
+
+Observation generators:
+
+* ``euclid10``: rows of Z are i.i.d. N(0, I_10); delta_ij = ||z_i - z_j||_2,
+  mirrored, zero diagonal.  Optional additive N(0, sigma^2) noise drawn once
+  per i < j and clipped at 0, and a symmetric missing (NaN) fraction.
+* ``chisq``: delta_ij = z^2, z ~ N(0, 1) per i < j, as in the reference
+  benchmark (src/benchmark.cpp:176-189), with its optional 25% missingness.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+DATA_SEED = 666  # src/benchmark.cpp:63, R/Untitled.R:546
+LOCATION_SEED = 667
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def _stream_state(seed: int, key: int) -> np.uint64:
+    s = np.array([(int(seed) * 0x100000001B3 + int(key) * 0xC2B2AE3D27D4EB4F) & 0xFFFFFFFFFFFFFFFF],
+                 dtype=np.uint64)
+    return _mix(s)[0]
+
+
+def uniforms(seed: int, key: int, count: int, offset: int = 0) -> np.ndarray:
+    """``count`` uniforms in (0, 1) from stream (seed, key), starting at ``offset``."""
+    state = _stream_state(seed, key)
+    with np.errstate(over="ignore"):
+        idx = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
+        z = _mix(state + idx * _GOLDEN)
+    return ((z >> np.uint64(11)).astype(np.float64) + 0.5) * (2.0 ** -53)
+
+
+def normals(seed: int, key: int, count: int) -> np.ndarray:
+    """``count`` standard normals from stream (seed, key) by Box-Muller."""
+    pairs = (count + 1) // 2
+    u = uniforms(seed, key, 2 * pairs)
+    u1, u2 = u[0::2], u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    t = 2.0 * np.pi * u2
+    out = np.empty(2 * pairs)
+    out[0::2] = r * np.cos(t)
+    out[1::2] = r * np.sin(t)
+    return out[:count]
+
+
+def locations(n: int, p: int, seed: int = LOCATION_SEED, key: int = 1) -> np.ndarray:
+    """Row-major N x P latent locations drawn from N(0, I_P)."""
+    return normals(seed, key, n * p).reshape(n, p)
+
+
+def _upper_pairs_mask(n: int, frac: float, seed: int, key: int) -> np.ndarray:
+    """Symmetric boolean N x N matrix: True for i != j pairs drawn missing."""
+    iu = np.triu_indices(n, 1)
+    u = uniforms(seed, key, iu[0].size)
+    m = np.zeros((n, n), dtype=bool)
+    m[iu] = u < frac
+    return m | m.T
+
+
+def euclid10(n: int, seed: int = DATA_SEED, noise_sd: float = 0.0, missing: float = 0.0,
+             dim: int = 10, block: int = 512) -> np.ndarray:
+    """Row-major N x N observations: distances of N(0, I_dim) points.
+
+    With ``noise_sd > 0`` each i < j distance gets N(0, noise_sd^2) noise (drawn
+    once, mirrored) and is clipped at 0; ``missing`` is the fraction of i < j
+    pairs set to NaN on both sides.
+    """
+    z = normals(seed, 1, n * dim).reshape(n, dim)
+    delta = np.empty((n, n))
+    for r0 in range(0, n, block):
+        r1 = min(n, r0 + block)
+        diff = z[r0:r1, None, :] - z[None, :, :]
+        delta[r0:r1] = np.sqrt(np.sum(diff * diff, axis=2))
+    np.fill_diagonal(delta, 0.0)
+    if noise_sd > 0.0:
+        iu = np.triu_indices(n, 1)
+        eps = noise_sd * normals(seed, 2, iu[0].size)
+        upper = np.maximum(delta[iu] + eps, 0.0)
+        delta[iu] = upper
+        delta.T[iu] = upper
+    if missing > 0.0:
+        delta[_upper_pairs_mask(n, missing, seed, 3)] = np.nan
+    return delta
+
+
+def chisq(n: int, seed: int = DATA_SEED, missing: bool = False) -> np.ndarray:
+    """Reference-benchmark style observations: z^2 per i < j (src/benchmark.cpp:176-189)."""
+    iu = np.triu_indices(n, 1)
+    z = normals(seed, 4, iu[0].size)
+    delta = np.zeros((n, n))
+    delta[iu] = z * z
+    delta.T[iu] = z * z
+    if missing:
+        delta[_upper_pairs_mask(n, 0.25, seed, 5)] = np.nan
+    return delta
+
+
+def config(name: str):
+    """Inputs of the named workload: (delta, X, precision, truncation)."""
+    if name == "n1000_p2":  # reference CPU-runnable case
+        n, p = 1000, 2
+        return euclid10(n), locations(n, p), 1.0, True
+    if name == "n10000_p2":  # scale + precision case
+        n, p = 10000, 2
+        return euclid10(n, noise_sd=0.1, missing=0.05), locations(n, p), 2.0, True
+    raise KeyError(name)

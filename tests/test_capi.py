@@ -1,0 +1,84 @@
+"""CPU checks of the drop-in boundary: libmds_hip.so loads, exports every
+function include/mds_hip.h declares, and fails cleanly (status + message, no
+crash, no CPU fallback) where no GPU is present."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from massivemds_amd import engine as eng
+from massivemds_amd import mds
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared(header):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mds_hip_\w+)\s*\(", text)))
+
+
+def _lib():
+    if not eng.library_path().exists():
+        from massivemds_amd import build
+
+        build.build()
+    return ctypes.CDLL(str(eng.library_path()))
+
+
+def test_header_symbols_exported():
+    lib = _lib()
+    names = _declared("mds_hip.h")
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_flags_match_reference_bits():
+    # src/flags.h:6-14
+    assert eng.Flags.FLOAT == 4 and eng.Flags.TBB == 8 and eng.Flags.OPENCL == 16
+    assert eng.Flags.LEFT_TRUNCATION == 32 and eng.Flags.SSE == 128
+    assert eng.Flags.AVX == 256 and eng.Flags.AVX512 == 512
+    text = (ROOT / "include" / "mds_hip.h").read_text()
+    for name, bit in [("FLOAT", 2), ("TBB", 3), ("OPENCL", 4), ("LEFT_TRUNCATION", 5), ("SSE", 7),
+                      ("AVX", 8), ("AVX512", 9), ("HIP", 10)]:
+        assert re.search(rf"MDS_HIP_FLAG_{name} \(1L << {bit}\)", text), name
+
+
+def test_max_dimension():
+    lib = eng.load_library()
+    assert lib.mds_hip_max_dimension() == 16
+
+
+def test_create_without_gpu_reports_error():
+    lib = eng.load_library()
+    if eng.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(eng.MdsError) as info:
+        eng.MultiDimensionalScaling(2, 10, eng.Flags.OPENCL)
+    assert info.value.status == 2
+    assert "device" in str(info.value).lower()
+
+
+def test_invalid_arguments_rejected_before_device():
+    for p in (0, 17):
+        with pytest.raises(eng.MdsError) as info:
+            eng.MultiDimensionalScaling(p, 10, eng.Flags.OPENCL)
+        assert info.value.status == 1
+    with pytest.raises(eng.MdsError):
+        eng.MultiDimensionalScaling(2, 0, eng.Flags.OPENCL)
+
+
+def test_null_handle_calls_fail():
+    lib = eng.load_library()
+    assert lib.mds_hip_store_state(None) == 1
+    assert lib.mds_hip_destroy(None) == 1
+    assert b"null" in lib.mds_hip_last_error(None)
+
+
+def test_cpu_routes_are_not_served():
+    with pytest.raises(ValueError):
+        eng.factory(2, 10, eng.Flags.TBB, 0, 4)
+    with pytest.raises(ValueError):
+        mds.createEngine(2, 10, truncation=True, gpu=0)
