@@ -17,6 +17,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libmds_hip.so"
+MATHCHECK_LIB = PKG / "libmds_mathcheck.so"
 BIN = PKG / "bin"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -64,6 +65,11 @@ def build(jobs: int = 8) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
     os.replace(tmp, LIB)
+    # host build of the kernels' arithmetic, for CPU accuracy tests
+    mobj = _compile(CSRC / "mds_mathcheck.hip")
+    tmp = MATHCHECK_LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(mobj), "-o", str(tmp)])
+    os.replace(tmp, MATHCHECK_LIB)
     return LIB
 
 

@@ -1,17 +1,15 @@
 // Device-side building blocks for the gfx950 MDS kernels: wavefront
-// reductions on DPP (no LDS, no atomics), compensated accumulators and the
-// per-pair truncated-normal terms.
+// reductions and rotations on DPP (no LDS, no atomics) and compensated
+// accumulators.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "mds_internal.hpp"
+#include "mds_trunc_math.hpp"
 
 namespace mdship {
-
-constexpr double kInvSqrt2Pi = 0.398942280401432677939946059934;  // 1 / sqrt(2 pi)
-constexpr double kSqrtHalf = 0.707106781186547524400844362105;    // 1 / sqrt(2)
 
 // DPP control words (GFX9 encoding, valid on gfx950).
 constexpr int kDppQuadXor1 = 0xB1;       // quad_perm [1,0,3,2]
@@ -84,39 +82,36 @@ struct LaneSum<float> {
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
-// Math wrappers resolving to the ocml routine of the matching precision.
-__device__ __forceinline__ double m_sqrt(double v) { return sqrt(v); }
-__device__ __forceinline__ float m_sqrt(float v) { return sqrtf(v); }
-__device__ __forceinline__ double m_erfc(double v) { return erfc(v); }
-__device__ __forceinline__ float m_erfc(float v) { return erfcf(v); }
-__device__ __forceinline__ double m_log1p(double v) { return log1p(v); }
-__device__ __forceinline__ float m_log1p(float v) { return log1pf(v); }
-__device__ __forceinline__ double m_exp(double v) { return exp(v); }
-__device__ __forceinline__ float m_exp(float v) { return expf(v); }
 __device__ __forceinline__ bool m_isnan(double v) { return v != v; }
 __device__ __forceinline__ bool m_isnan(float v) { return v != v; }
 
-// Truncation terms at z = d * sqrt(tau) >= 0, with Q(z) = erfc(z / sqrt 2) / 2:
-//   log Phi(z) = log1p(-Q)                              (likelihood)
-//   phi(z) / (Phi(z) sqrt(tau))                         (gradient)
-// The reference evaluates log(erfc(-z / sqrt 2) / 2) and recovers Phi with
-// exp() of it (src/CDF.h:583-592, src/NewMultiDimensionalScaling.hpp:439-441);
-// the forms here are algebraically identical, computed once per unordered
-// pair and accurate to a few ulp as Phi -> 1.
-template <typename Real>
-struct TruncTerms {
-    Real logPhi;
-    Real gradTerm;
-};
+// Column accumulators of the skewed tile schedule.  At step k lane l holds
+// the partial sum of tile column (l + k) mod 64; rotate() moves every
+// accumulator one lane down (DPP wave_rol:1, lane l <- lane l + 1) so that
+// at step k + 1 lane l holds column (l + k + 1) mod 64.  After 64 steps lane
+// l holds column l.
+constexpr int kDppWaveRol1 = 0x134;
 
 template <typename Real>
-__device__ __forceinline__ TruncTerms<Real> trunc_terms(Real z, Real sqrtTau) {
-    const Real q = Real(0.5) * m_erfc(z * Real(kSqrtHalf));
-    const Real phi = Real(kInvSqrt2Pi) * m_exp(Real(-0.5) * z * z);
-    TruncTerms<Real> t;
-    t.logPhi = m_log1p(-q);
-    t.gradTerm = phi / ((Real(1) - q) * sqrtTau);
-    return t;
-}
+struct ColSum;
+
+template <>
+struct ColSum<double> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ void rotate() { s = dpp_mov<kDppWaveRol1, 0xf>(s); }
+    __device__ __forceinline__ double value() const { return s; }
+};
+
+template <>
+struct ColSum<float> {
+    LaneSum<float> k;
+    __device__ __forceinline__ void add(float v) { k.add(v); }
+    __device__ __forceinline__ void rotate() {
+        k.s = dpp_mov<kDppWaveRol1, 0xf>(k.s);
+        k.c = dpp_mov<kDppWaveRol1, 0xf>(k.c);
+    }
+    __device__ __forceinline__ double value() const { return k.value(); }
+};
 
 }  // namespace mdship

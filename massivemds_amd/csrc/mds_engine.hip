@@ -81,11 +81,12 @@ static bool same_value(Real a, Real b) {
     return (a != a && b != b) || std::memcmp(&a, &b, sizeof(Real)) == 0;
 }
 
-// Packs the row-major N x N matrix into column-major 64 x 64 tiles of the
-// upper block triangle.  plane0[t] holds delta(64I + il, 64J + jl) and
-// plane1[t] the transposed tile delta(64J + jl, 64I + il); entries outside
+// Packs the row-major N x N matrix into 64 x 64 tiles of the upper block
+// triangle in the kernels' skewed order: element k * 64 + l of tile (I, J)
+// holds delta(64 I + l, 64 J + (l + k) mod 64) in plane 0 and the transposed
+// entry delta(64 J + (l + k) mod 64, 64 I + l) in plane 1.  Entries outside
 // the matrix are NaN (missing).  Returns whether the stored values are
-// symmetric, in which case plane1 is not needed.
+// symmetric, in which case plane 1 is not needed.
 template <typename Real>
 static bool pack_tiles(const double* src, int n, const std::vector<uint32_t>& tiles, Real* plane0,
                        Real* plane1) {
@@ -97,20 +98,23 @@ static bool pack_tiles(const double* src, int n, const std::vector<uint32_t>& ti
             const int I = (int)(tiles[t] >> 16), J = (int)(tiles[t] & 0xffffu);
             Real* a = plane0 + t * kTileElems;
             Real* b = plane1 + t * kTileElems;
-            for (int il = 0; il < kTile; ++il) {
-                const int64_t i = (int64_t)I * kTile + il;
-                for (int jl = 0; jl < kTile; ++jl) {
-                    const int64_t j = (int64_t)J * kTile + jl;
-                    a[jl * kTile + il] = (i < n && j < n) ? (Real)src[i * n + j] : nan;
+            // plane 0: row i fixed, columns visited in order (contiguous reads)
+            for (int l = 0; l < kTile; ++l) {
+                const int64_t i = (int64_t)I * kTile + l;
+                for (int k = 0; k < kTile; ++k) {
+                    const int64_t j = (int64_t)J * kTile + ((l + k) & (kTile - 1));
+                    a[k * kTile + l] = (i < n && j < n) ? (Real)src[i * n + j] : nan;
                 }
             }
-            for (int jl = 0; jl < kTile; ++jl) {
-                const int64_t j = (int64_t)J * kTile + jl;
-                for (int il = 0; il < kTile; ++il) {
-                    const int64_t i = (int64_t)I * kTile + il;
+            // plane 1: source row j fixed, l varies (contiguous reads)
+            for (int c = 0; c < kTile; ++c) {
+                const int64_t j = (int64_t)J * kTile + c;
+                for (int l = 0; l < kTile; ++l) {
+                    const int64_t i = (int64_t)I * kTile + l;
+                    const int k = (c - l) & (kTile - 1);
                     const Real v = (i < n && j < n) ? (Real)src[j * n + i] : nan;
-                    b[jl * kTile + il] = v;
-                    if (i < j && !same_value(v, a[jl * kTile + il])) local = true;
+                    b[k * kTile + l] = v;
+                    if (i < j && !same_value(v, a[k * kTile + l])) local = true;
                 }
             }
         }

@@ -3,16 +3,16 @@
 //
 // Work decomposition.  The N x N observation matrix is stored by the engine
 // as packed 64 x 64 tiles of its upper block triangle (tile (I, J), I <= J),
-// each tile column-major so that one wavefront reads one tile column (64
-// consecutive values) per load.  Symmetric data keep one plane; asymmetric
+// each tile in skewed order (step k, lane l) -> (row l, column (l + k) mod 64)
+// so that one wavefront reads 64 consecutive values per step.  Symmetric data keep one plane; asymmetric
 // data keep a second plane with the transposed tile (delta_ji), so the
 // reference's row-wise semantics (row i uses delta_ij, row j uses delta_ji;
 // src/NewMultiDimensionalScaling.hpp:421-461, 494-522) are reproduced while
 // every transcendental is evaluated once per unordered pair.
 //
-// One wavefront owns one tile: lane l is row i = 64 I + l and loops over the
-// 64 columns j.  Row partials of the gradient stay in registers; column
-// partials are reduced across the wavefront with DPP and kept by lane j.
+// One wavefront owns one tile: lane l is row i = 64 I + l and visits the 64
+// columns in 64 steps.  Row partials of the gradient stay in registers;
+// column partials rotate across the wavefront with DPP.
 // Each tile writes its likelihood partial and its two 64-row gradient
 // partials to fixed slots; a second kernel adds them in a fixed order in
 // FP64.  No atomics anywhere, so results are bitwise reproducible.
@@ -21,11 +21,35 @@
 
 namespace mdship {
 
+template <typename Real>
+__device__ __forceinline__ Trunc<Real> trunc_terms(Real d, Real d2, Real halfTau, Real sqrtTau, Real cg);
+
+template <>
+__device__ __forceinline__ Trunc<double> trunc_terms(double d, double d2, double halfTau, double sqrtTau,
+                                                     double cg) {
+    return trunc64<HwSeeds>(d, d2, halfTau, sqrtTau, cg);
+}
+
+template <>
+__device__ __forceinline__ Trunc<float> trunc_terms(float d, float d2, float halfTau, float sqrtTau, float cg) {
+    return trunc32<HwSeeds>(d, d2, halfTau, sqrtTau, cg);
+}
+
+__device__ __forceinline__ void distance(double d2, double& d, double& invd) {
+    distance64<HwSeeds>(d2, d, invd);
+}
+__device__ __forceinline__ void distance(float d2, float& d, float& invd) { distance32<HwSeeds>(d2, d, invd); }
+
+// One wavefront per tile.  Skewed schedule: at step k lane l evaluates the
+// pair (row 64 I + l, column 64 J + (l + k) mod 64); the engine stores each
+// tile in that order, so every step is one coalesced 64-value load.  Row
+// partials stay in the lane; column partials rotate one lane per step and
+// end in the lane of their column.
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
     const Real* __restrict__ delta, const Real* __restrict__ x,
     const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
-    Real tau, Real halfTau, Real sqrtTau, double* __restrict__ tileSums,
+    Real tau, Real halfTau, Real sqrtTau, Real cg, double* __restrict__ tileSums,
     double* __restrict__ part) {
     const int lane = threadIdx.x % kWave;
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
@@ -42,20 +66,20 @@ __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
 
     LaneSum<Real> sAcc;
     LaneSum<Real> rowAcc[P];
-    Real colAcc[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) colAcc[p] = Real(0);
+    ColSum<Real> colAcc[P];
 
-    const Real* tileCol = delta + (int64_t)t * kTileElems + lane;
-    const Real* xcol = x + (int64_t)J * kTile * P;
+    const Real* tileStep = delta + (int64_t)t * kTileElems + lane;
+    const Real* xJ = x + (int64_t)J * kTile * P;
 
-    for (int jl = 0; jl < kTile; ++jl) {
+    for (int k = 0; k < kTile; ++k) {
+        const int c = (lane + k) & (kTile - 1);
         Real xj[P];
 #pragma unroll
-        for (int p = 0; p < P; ++p) xj[p] = xcol[jl * P + p];
-        const Real dij = tileCol[jl * kTile];
-        const Real dji = SYM ? dij : tileCol[planeStride + jl * kTile];
-        const bool inTri = !diag || lane < jl;
+        for (int p = 0; p < P; ++p) xj[p] = xJ[c * P + p];
+        const Real dij = tileStep[k * kTile];
+        const Real dji = SYM ? dij : tileStep[planeStride + k * kTile];
+        // diagonal tiles keep i < j only: lane < c, i.e. 0 < k < 64 - lane
+        const bool inTri = !diag || (k > 0 && lane + k < kTile);
         const bool vij = inTri && !m_isnan(dij);
         const bool vji = inTri && !m_isnan(dji);
 
@@ -64,16 +88,17 @@ __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             diff[p] = xi[p] - xj[p];
-            d2 += diff[p] * diff[p];
+            d2 = fma(diff[p], diff[p], d2);
         }
-        const Real d = m_sqrt(d2);
+        Real d, invd;
+        distance(d2, d, invd);
         const Real rij = dij - d;
         const Real rji = dji - d;
 
         Real lij, lji;
         Real gt = Real(0);
         if (TRUNC) {
-            const TruncTerms<Real> tt = trunc_terms<Real>(d * sqrtTau, sqrtTau);
+            const Trunc<Real> tt = trunc_terms<Real>(d, d2, halfTau, sqrtTau, cg);
             lij = halfTau * rij * rij + tt.logPhi;
             lji = halfTau * rji * rji + tt.logPhi;
             gt = tt.gradTerm;
@@ -89,16 +114,15 @@ __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
         if (GRAD) {
             // c = (r - phi / (Phi sqrt tau)) * tau / d for each ordered pair
             // (src/NewMultiDimensionalScaling.hpp:435-444); masked pairs give 0.
-            const Real tauOverD = tau / d;
+            const Real tauOverD = tau * invd;
             const Real cij = vij ? (rij - gt) * tauOverD : Real(0);
             const Real cji = SYM ? cij : (vji ? (rji - gt) * tauOverD : Real(0));
 #pragma unroll
             for (int p = 0; p < P; ++p) {
                 const Real rowTerm = cij * diff[p];
                 rowAcc[p].add(rowTerm);
-                const Real colTerm = SYM ? -rowTerm : -(cji * diff[p]);
-                const Real col = read_last_lane(wave_sum_to_last(colTerm));
-                colAcc[p] = (lane == jl) ? col : colAcc[p];
+                colAcc[p].add(SYM ? -rowTerm : -(cji * diff[p]));
+                colAcc[p].rotate();
             }
         }
     }
@@ -110,13 +134,13 @@ __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
         double* rowSlot = part + ((int64_t)I * nb + J) * (P * kTile) + lane;
         if (diag) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) rowSlot[p * kTile] = rowAcc[p].value() + (double)colAcc[p];
+            for (int p = 0; p < P; ++p) rowSlot[p * kTile] = rowAcc[p].value() + colAcc[p].value();
         } else {
             double* colSlot = part + ((int64_t)J * nb + I) * (P * kTile) + lane;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
                 rowSlot[p * kTile] = rowAcc[p].value();
-                colSlot[p * kTile] = (double)colAcc[p];
+                colSlot[p * kTile] = colAcc[p].value();
             }
         }
     }
@@ -180,9 +204,10 @@ static hipError_t go(const TileLaunch& a, hipStream_t s) {
     const Real tau = (Real)a.tau;
     const Real halfTau = (Real)(0.5 * a.tau);
     const Real sqrtTau = (Real)sqrt(a.tau);
+    const Real cg = (Real)(kInvSqrt2Pi / sqrt(a.tau));
     hipLaunchKernelGGL((tile_kernel<Real, P, TRUNC, SYM, GRAD>), grid, dim3(kBlockThreads), 0, s,
                        (const Real*)a.delta, (const Real*)a.x, a.tiles, a.numTiles, a.nb,
-                       a.planeStride, tau, halfTau, sqrtTau, a.tileSums, a.part);
+                       a.planeStride, tau, halfTau, sqrtTau, cg, a.tileSums, a.part);
     return hipGetLastError();
 }
 

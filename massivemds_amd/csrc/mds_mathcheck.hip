@@ -1,0 +1,52 @@
+// Host build of the kernels' fused truncation arithmetic (mds_trunc_math.hpp)
+// so that its accuracy can be checked on the CPU against high-precision
+// values.  The hardware reciprocal / reciprocal-square-root seeds are replaced
+// by float-precision seeds of the same (~2^-24) accuracy; the Newton steps,
+// polynomials and rational forms are the ones the device runs.
+#include <cmath>
+
+#include "mds_trunc_math.hpp"
+
+namespace {
+struct SoftSeeds {
+    static double rcp(double x) { return (double)(1.0f / (float)x); }
+    static double rsq(double x) { return (double)(1.0f / std::sqrt((float)x)); }
+    static float rcp(float x) { return 1.0f / x; }
+    static float rsq(float x) { return 1.0f / std::sqrt(x); }
+    static float exp2(float x) { return std::exp2(x); }
+};
+}  // namespace
+
+extern "C" {
+
+// For each squared distance d2[k]: d, 1/d, log Phi(d sqrt tau) and
+// phi / (Phi sqrt tau), FP64 forms.
+void mds_mathcheck_f64(const double* d2, long long n, double tau, double* d, double* invd,
+                       double* logPhi, double* gradTerm) {
+    const double halfTau = 0.5 * tau, sqrtTau = std::sqrt(tau);
+    const double cg = mdship::kInvSqrt2Pi / sqrtTau;
+    for (long long k = 0; k < n; ++k) {
+        mdship::distance64<SoftSeeds>(d2[k], d[k], invd[k]);
+        const mdship::Trunc<double> t = mdship::trunc64<SoftSeeds>(d[k], d2[k], halfTau, sqrtTau, cg);
+        logPhi[k] = t.logPhi;
+        gradTerm[k] = t.gradTerm;
+    }
+}
+
+// FP32 forms, results widened to double.
+void mds_mathcheck_f32(const double* d2, long long n, double tau, double* d, double* invd,
+                       double* logPhi, double* gradTerm) {
+    const float halfTau = (float)(0.5 * tau), sqrtTau = (float)std::sqrt(tau);
+    const float cg = (float)(mdship::kInvSqrt2Pi / std::sqrt(tau));
+    for (long long k = 0; k < n; ++k) {
+        float df, idf;
+        const float d2f = (float)d2[k];
+        mdship::distance32<SoftSeeds>(d2f, df, idf);
+        const mdship::Trunc<float> t = mdship::trunc32<SoftSeeds>(df, d2f, halfTau, sqrtTau, cg);
+        d[k] = df;
+        invd[k] = idf;
+        logPhi[k] = t.logPhi;
+        gradTerm[k] = t.gradTerm;
+    }
+}
+}

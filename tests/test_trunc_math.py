@@ -1,0 +1,94 @@
+"""Accuracy of the kernels' fused truncation arithmetic, checked on the CPU.
+
+libmds_mathcheck.so compiles mds_trunc_math.hpp for the host (hardware seeds
+replaced by float-precision seeds of the same accuracy).  Its log Phi(z),
+phi(z) / (Phi(z) sqrt tau), d and 1/d are compared with mpmath at 40 digits
+over the whole range of z that matters (and beyond, where the terms vanish).
+
+Budget: the FP64 engine must match the reference to 1e-12 relative on sums
+of up to ~1e8 terms, so each FP64 term needs ~1e-14 relative accuracy; the
+bounds below are two orders tighter.
+"""
+import ctypes
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from massivemds_amd import build as hipbuild
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _lib():
+    if not hipbuild.MATHCHECK_LIB.exists():
+        hipbuild.build()
+    lib = ctypes.CDLL(str(hipbuild.MATHCHECK_LIB))
+    for name in ("mds_mathcheck_f64", "mds_mathcheck_f32"):
+        fn = getattr(lib, name)
+        fn.argtypes = [_dp, ctypes.c_longlong, ctypes.c_double, _dp, _dp, _dp, _dp]
+        fn.restype = None
+    return lib
+
+
+def _eval(fp32, d2, tau):
+    lib = _lib()
+    d2 = np.ascontiguousarray(d2, dtype=np.float64)
+    outs = [np.empty_like(d2) for _ in range(4)]
+    fn = lib.mds_mathcheck_f32 if fp32 else lib.mds_mathcheck_f64
+    fn(d2.ctypes.data_as(_dp), d2.size, tau, *[o.ctypes.data_as(_dp) for o in outs])
+    return outs
+
+
+def _exact(d2, tau):
+    mp.mp.dps = 40
+    d = mp.sqrt(mp.mpf(d2))
+    z = d * mp.sqrt(tau)
+    phi_cdf = mp.ncdf(z)
+    log_phi = mp.log(phi_cdf)
+    grad = mp.npdf(z) / (phi_cdf * mp.sqrt(tau))
+    return float(d), float(log_phi), float(grad)
+
+
+def _grid(tau):
+    z = np.concatenate([np.linspace(0.0, 1.0, 101), np.linspace(1.0, 9.0, 401), np.linspace(9.0, 40.0, 63),
+                        np.array([1e-8, 1e-4, 0.3, 2.5, 12.0, 11.999, 12.001, 37.0, 100.0])])
+    return (z / np.sqrt(tau)) ** 2
+
+
+@pytest.mark.parametrize("tau", [0.5, 1.0, 2.0, 3.7])
+def test_fp64_terms(tau):
+    d2 = _grid(tau)
+    d, invd, log_phi, grad = _eval(False, d2, tau)
+    for k, v in enumerate(d2):
+        ed, elog, egrad = _exact(v, tau)
+        assert abs(d[k] - ed) <= 4e-16 * ed, (v, d[k], ed)
+        if v > 0:
+            assert abs(invd[k] * ed - 1) <= 4e-16
+        # log Phi in [-ln 2, 0]: absolute error, as it enters a sum of O(1) terms
+        assert abs(log_phi[k] - elog) <= 1e-15, (v, log_phi[k], elog)
+        # exp(-z^2/2) carries the rounding of its argument: relative error
+        # ~ (z^2/2) eps, as in any evaluation from a rounded z (the reference's
+        # exp(-0.5 z z) included); below 1e-290 the value is denormal and nil
+        w = 0.5 * v * tau
+        assert abs(grad[k] - egrad) <= (1e-15 + 3e-16 * w) * egrad + 1e-290, (v, grad[k], egrad)
+
+
+def test_fp64_zero_distance():
+    d, invd, log_phi, grad = _eval(False, np.array([0.0]), 2.0)
+    assert d[0] == 0.0 and np.isinf(invd[0])
+    assert abs(log_phi[0] - np.log(0.5)) <= 5e-16
+    assert abs(grad[0] - 2 * 0.3989422804014327 / np.sqrt(2.0)) <= 1e-15
+
+
+@pytest.mark.parametrize("tau", [0.5, 2.0])
+def test_fp32_terms(tau):
+    d2 = _grid(tau)
+    d, invd, log_phi, grad = _eval(True, d2, tau)
+    for k, v in enumerate(d2):
+        ed, elog, egrad = _exact(np.float64(np.float32(v)), tau)
+        assert abs(d[k] - ed) <= 3e-7 * ed
+        assert abs(log_phi[k] - elog) <= 2e-7, (v, log_phi[k], elog)  # absolute, as FP64 above
+        z2 = v * tau
+        # exp2 of a rounded argument: relative error grows with z^2 / 2
+        assert abs(grad[k] - egrad) <= (1e-6 + 2e-7 * z2) * egrad + 1e-37, (v, grad[k], egrad)
