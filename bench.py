@@ -114,6 +114,69 @@ def _cpu_baseline(n, p, tau, delta, x, budget_s=20.0):
     }
 
 
+def _bench_batch(steps, warmup, count=64, tune_cache=None):
+    """Uneven batch: `count` datasets, N log-uniform in [500, 8000], P in
+    {2, 5, 10}, tau in [0.5, 4], truncation, FP32 with FP64 reductions; one
+    persistent launch per evaluation against `count` sequential
+    single-instance evaluations of the same data."""
+    from massivemds_amd import datagen
+    from massivemds_amd.batch import MultiDimensionalScalingBatch
+    from massivemds_amd.engine import Flags, MultiDimensionalScaling
+
+    flags = Flags.OPENCL | Flags.LEFT_TRUNCATION | Flags.FLOAT
+    specs = datagen.batch_specs(count)
+    b = MultiDimensionalScalingBatch([s[1] for s in specs], [s[0] for s in specs], flags, 0)
+    singles = []
+    for k, (n, p, tau, seed) in enumerate(specs):
+        delta, x = datagen.batch_dataset(n, p, seed)
+        b.setPairwiseData(k, delta)
+        b.updateLocations(k, x)
+        b.setParameters(k, [tau])
+        e = MultiDimensionalScaling(p, n, flags, 0)
+        e.setPairwiseData(delta)
+        e.updateLocations(-1, x)
+        e.setParameters([tau])
+        singles.append(e)
+        del delta
+    t0 = time.perf_counter()
+    tuned_ms = b.autotune(tune_cache)
+    tune_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        b.evaluate()
+    b.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.evaluate()
+    wall_b = (time.perf_counter() - t0) / steps
+    ev_ms, cnt = b.get_timing()
+    sums = b.sums()
+    for e in singles:
+        e.evaluate_async(True)
+        e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for e in singles:
+            e.evaluate_async(True)
+            e.synchronize()
+    wall_s = (time.perf_counter() - t0) / steps
+    seq = np.array([e.getSumOfIncrements() for e in singles])
+    identical = bool(np.array_equal(sums, seq))
+    pairs = sum(n * (n - 1) // 2 for n, _, _, _ in specs)
+    algo = sum(4 * n * (n - 1) // 2 + 2 * 8 * n * p for n, p, _, _ in specs)
+    out = {
+        "workload": f"batch{count}_uneven_fp32_trunc", "datasets": count, "pairs": pairs,
+        "batch_ms": ev_ms / cnt, "batch_wall_ms": wall_b * 1e3, "sequential_wall_ms": wall_s * 1e3,
+        "speedup_vs_sequential": wall_s / wall_b, "batch_evals_per_s": 1.0 / wall_b,
+        "achieved_GBs": algo / (ev_ms / cnt * 1e-3) / 1e9, "algorithmic_bytes": algo,
+        "config": dict(zip(["wavesPerBlock", "blocksPerCU", "tilesPerItem", "strategy"], b.get_config())),
+        "autotune_s": tune_s, "autotune_best_ms": tuned_ms, "bitwise_equal_to_sequential": identical,
+    }
+    for e in singles:
+        e.close()
+    b.close()
+    print(json.dumps(out))
+
+
 def _calibrate_copy(mib):
     """A plain elementwise copy kernel of known size (torch, as plumbing)."""
     import torch
@@ -135,12 +198,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--workload", default="n10000", choices=["n10000", "batch64"],
+                    help="n10000: the metric's workload (default); batch64: the uneven batch")
+    ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch64)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
     if args.calibrate_copy:
         _calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload == "batch64":
+        _bench_batch(args.steps, args.warmup, tune_cache=args.tune_cache)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a "

@@ -125,6 +125,40 @@ int mds_hip_get_timing(mds_hip_handle h, double* tileMs, double* evalMs, long lo
 int mds_hip_get_layout(mds_hip_handle h, int* symmetric, long long* observationBytes,
                        long long* tiles);
 
+/*
+ * Batched evaluation (no counterpart in the reference, whose instances are
+ * independent objects driven one call at a time; for callers with many
+ * independent datasets or chains).  `count` datasets of their own N and P
+ * (P <= 10) share the flags (FLOAT, LEFT_TRUNCATION); each has its own data,
+ * locations and precision, with the same argument meaning as the
+ * single-instance calls.  mds_hip_batch_evaluate runs every dataset's
+ * likelihood and gradient in one persistent launch plus one reduce launch.
+ */
+typedef struct mds_hip_batch* mds_hip_batch_handle;
+
+int mds_hip_batch_create(int count, const int* embeddingDimensions, const int* locationCounts, long flags,
+                         int device, mds_hip_batch_handle* out);
+int mds_hip_batch_destroy(mds_hip_batch_handle h);
+const char* mds_hip_batch_last_error(mds_hip_batch_handle h);
+int mds_hip_batch_set_pairwise_data(mds_hip_batch_handle h, int index, const double* data, size_t length);
+int mds_hip_batch_update_locations(mds_hip_batch_handle h, int index, const double* locations, size_t length);
+int mds_hip_batch_set_parameters(mds_hip_batch_handle h, int index, const double* data, size_t length);
+int mds_hip_batch_evaluate(mds_hip_batch_handle h);
+/* getSumOfIncrements of every dataset (length = count). */
+int mds_hip_batch_get_sums_of_increments(mds_hip_batch_handle h, double* out, size_t length);
+int mds_hip_batch_get_gradient(mds_hip_batch_handle h, int index, double* out, size_t length);
+/* Times every launch configuration on this batch and keeps the fastest;
+ * with cachePath, reuses a cached choice for the same batch signature
+ * (unless force) and appends new choices.  bestMs: its time per evaluation. */
+int mds_hip_batch_autotune(mds_hip_batch_handle h, const char* cachePath, int force, double* bestMs);
+/* strategy 0: one launch over all items; 1: one launch per P class. */
+int mds_hip_batch_set_config(mds_hip_batch_handle h, int wavesPerBlock, int blocksPerCU, int tilesPerItem,
+                             int strategy);
+int mds_hip_batch_get_config(mds_hip_batch_handle h, int* wavesPerBlock, int* blocksPerCU, int* tilesPerItem,
+                             int* strategy);
+int mds_hip_batch_set_timing(mds_hip_batch_handle h, int enable);
+int mds_hip_batch_get_timing(mds_hip_batch_handle h, double* ms, long long* count);
+
 #ifdef __cplusplus
 }
 #endif

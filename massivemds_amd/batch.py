@@ -1,0 +1,128 @@
+"""Batched engine: many independent datasets of their own N and P evaluated
+in one persistent launch (C ABI: mds_hip_batch_* in include/mds_hip.h).
+
+Each dataset keeps the single-instance semantics: its own observations,
+locations and precision; `sums()` returns every dataset's
+getSumOfIncrements() and `gradient(k)` the N_k x P_k log-likelihood gradient.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .engine import MdsError, _as_f64, _c_double_p, _ptr, load_library
+
+_bound = False
+
+
+def _bind(lib):
+    global _bound
+    if _bound:
+        return lib
+    H = ctypes.c_void_p
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "mds_hip_batch_create": ([ctypes.c_int, ip, ip, ctypes.c_long, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "mds_hip_batch_destroy": ([H], ctypes.c_int),
+        "mds_hip_batch_last_error": ([H], ctypes.c_char_p),
+        "mds_hip_batch_set_pairwise_data": ([H, ctypes.c_int, _c_double_p, ctypes.c_size_t], ctypes.c_int),
+        "mds_hip_batch_update_locations": ([H, ctypes.c_int, _c_double_p, ctypes.c_size_t], ctypes.c_int),
+        "mds_hip_batch_set_parameters": ([H, ctypes.c_int, _c_double_p, ctypes.c_size_t], ctypes.c_int),
+        "mds_hip_batch_evaluate": ([H], ctypes.c_int),
+        "mds_hip_batch_get_sums_of_increments": ([H, _c_double_p, ctypes.c_size_t], ctypes.c_int),
+        "mds_hip_batch_get_gradient": ([H, ctypes.c_int, _c_double_p, ctypes.c_size_t], ctypes.c_int),
+        "mds_hip_batch_autotune": ([H, ctypes.c_char_p, ctypes.c_int, _c_double_p], ctypes.c_int),
+        "mds_hip_batch_set_config": ([H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "mds_hip_batch_get_config": ([H, ip, ip, ip, ip], ctypes.c_int),
+        "mds_hip_batch_set_timing": ([H, ctypes.c_int], ctypes.c_int),
+        "mds_hip_batch_get_timing": ([H, _c_double_p, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _bound = True
+    return lib
+
+
+class MultiDimensionalScalingBatch:
+    def __init__(self, embeddingDimensions, locationCounts, flags: int, device: int = 0):
+        self._lib = _bind(load_library())
+        dims = np.ascontiguousarray(embeddingDimensions, dtype=np.int32)
+        ns = np.ascontiguousarray(locationCounts, dtype=np.int32)
+        if dims.size != ns.size:
+            raise ValueError("one embedding dimension per dataset")
+        h = ctypes.c_void_p()
+        ip = ctypes.POINTER(ctypes.c_int)
+        st = self._lib.mds_hip_batch_create(int(dims.size), dims.ctypes.data_as(ip), ns.ctypes.data_as(ip),
+                                            int(flags), int(device), ctypes.byref(h))
+        if st != 0:
+            raise MdsError(st, self._lib.mds_hip_batch_last_error(None).decode())
+        self._h = h
+        self.dims = [int(v) for v in dims]
+        self.ns = [int(v) for v in ns]
+
+    def _check(self, st):
+        if st != 0:
+            raise MdsError(st, self._lib.mds_hip_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mds_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setPairwiseData(self, index: int, data) -> None:
+        a = _as_f64(data)
+        self._check(self._lib.mds_hip_batch_set_pairwise_data(self._h, int(index), _ptr(a), a.size))
+
+    def updateLocations(self, index: int, locations) -> None:
+        a = _as_f64(locations)
+        self._check(self._lib.mds_hip_batch_update_locations(self._h, int(index), _ptr(a), a.size))
+
+    def setParameters(self, index: int, parameters) -> None:
+        a = _as_f64(parameters)
+        self._check(self._lib.mds_hip_batch_set_parameters(self._h, int(index), _ptr(a), a.size))
+
+    def evaluate(self) -> None:
+        self._check(self._lib.mds_hip_batch_evaluate(self._h))
+
+    def sums(self) -> np.ndarray:
+        out = np.empty(len(self.ns))
+        self._check(self._lib.mds_hip_batch_get_sums_of_increments(self._h, _ptr(out), out.size))
+        return out
+
+    def gradient(self, index: int) -> np.ndarray:
+        n, p = self.ns[index], self.dims[index]
+        out = np.empty(n * p)
+        self._check(self._lib.mds_hip_batch_get_gradient(self._h, int(index), _ptr(out), out.size))
+        return out.reshape(n, p)
+
+    def autotune(self, cache: str | None = None, force: bool = False) -> float:
+        ms = ctypes.c_double()
+        path = cache.encode() if cache else None
+        self._check(self._lib.mds_hip_batch_autotune(self._h, path, int(force), ctypes.byref(ms)))
+        return ms.value
+
+    def set_config(self, wavesPerBlock: int, blocksPerCU: int, tilesPerItem: int, strategy: int) -> None:
+        self._check(self._lib.mds_hip_batch_set_config(self._h, wavesPerBlock, blocksPerCU, tilesPerItem, strategy))
+
+    def get_config(self):
+        vals = [ctypes.c_int() for _ in range(4)]
+        self._check(self._lib.mds_hip_batch_get_config(self._h, *[ctypes.byref(v) for v in vals]))
+        return tuple(v.value for v in vals)
+
+    def set_timing(self, enable: bool) -> None:
+        self._check(self._lib.mds_hip_batch_set_timing(self._h, int(bool(enable))))
+
+    def get_timing(self):
+        ms, c = ctypes.c_double(), ctypes.c_longlong()
+        self._check(self._lib.mds_hip_batch_get_timing(self._h, ctypes.byref(ms), ctypes.byref(c)))
+        return ms.value, c.value

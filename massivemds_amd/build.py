@@ -36,7 +36,7 @@ HIP_FLAGS = [
 ]
 
 # The library: every .hip file under csrc/ except stand-alone programs.
-LIB_SOURCES = ["mds_kernels.hip", "mds_engine.hip"]
+LIB_SOURCES = ["mds_kernels.hip", "mds_engine.hip", "mds_batch.hip"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -70,7 +70,18 @@ def build(jobs: int = 8) -> Path:
     tmp = MATHCHECK_LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(mobj), "-o", str(tmp)])
     os.replace(tmp, MATHCHECK_LIB)
+    build_cli()
     return LIB
+
+
+def build_cli() -> Path:
+    """mds_benchmark: the command-line benchmark over mds::factory (host C++)."""
+    BIN.mkdir(exist_ok=True)
+    exe = BIN / "mds_benchmark"
+    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}",
+          f"-I{CSRC}", str(CSRC / "mds_factory.cpp"), str(CSRC / "mds_benchmark.cpp"),
+          f"-L{PKG}", "-lmds_hip", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)])
+    return exe
 
 
 def main() -> None:

@@ -16,176 +16,24 @@
 // Each tile writes its likelihood partial and its two 64-row gradient
 // partials to fixed slots; a second kernel adds them in a fixed order in
 // FP64.  No atomics anywhere, so results are bitwise reproducible.
-#include "mds_device.hpp"
 #include "mds_internal.hpp"
+#include "mds_tile.hpp"
 
 namespace mdship {
 
-template <typename Real>
-__device__ __forceinline__ Trunc<Real> trunc_terms(Real d, Real d2, Real halfTau, Real sqrtTau, Real cg);
-
-template <>
-__device__ __forceinline__ Trunc<double> trunc_terms(double d, double d2, double halfTau, double sqrtTau,
-                                                     double cg) {
-    return trunc64<HwSeeds>(d, d2, halfTau, sqrtTau, cg);
-}
-
-template <>
-__device__ __forceinline__ Trunc<float> trunc_terms(float d, float d2, float halfTau, float sqrtTau, float cg) {
-    return trunc32<HwSeeds>(d, d2, halfTau, sqrtTau, cg);
-}
-
-__device__ __forceinline__ void distance(double d2, double& d, double& invd) {
-    distance64<HwSeeds>(d2, d, invd);
-}
-__device__ __forceinline__ void distance(float d2, float& d, float& invd) { distance32<HwSeeds>(d2, d, invd); }
-
-// Per-wavefront accumulators of one tile.
-template <typename Real, int P>
-struct TileAcc {
-    LaneSum<Real> s;
-    LaneSum<Real> row[P];
-    ColSum<Real> col[P];
-};
-
-// One step of the skewed schedule for one lane: the pair (row i, column c)
-// with observations dij = delta_ij and dji = delta_ji.
-template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
-__device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi)[P], const Real (&xj)[P],
-                                          Real dij, Real dji, bool inTri, Real tau, Real halfTau,
-                                          Real sqrtTau, Real cg) {
-    const bool vij = inTri && !m_isnan(dij);
-    const bool vji = inTri && !m_isnan(dji);
-    Real diff[P];
-    Real d2 = Real(0);
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        diff[p] = xi[p] - xj[p];
-        d2 = fma(diff[p], diff[p], d2);
-    }
-    Real d, invd;
-    distance(d2, d, invd);
-    const Real rij = dij - d;
-    const Real rji = dji - d;
-
-    Real lij, lji;
-    Real gt = Real(0);
-    if (TRUNC) {
-        const Trunc<Real> tt = trunc_terms<Real>(d, d2, halfTau, sqrtTau, cg);
-        lij = halfTau * rij * rij + tt.logPhi;
-        lji = halfTau * rji * rji + tt.logPhi;
-        gt = tt.gradTerm;
-    } else {
-        lij = rij * rij;
-        lji = rji * rji;
-    }
-    if (SYM)
-        acc.s.add(vij ? lij + lij : Real(0));
-    else
-        acc.s.add((vij ? lij : Real(0)) + (vji ? lji : Real(0)));
-
-    if (GRAD) {
-        // c = (r - phi / (Phi sqrt tau)) * tau / d for each ordered pair
-        // (src/NewMultiDimensionalScaling.hpp:435-444); masked pairs give 0.
-        const Real tauOverD = tau * invd;
-        const Real cij = vij ? (rij - gt) * tauOverD : Real(0);
-        const Real cji = SYM ? cij : (vji ? (rji - gt) * tauOverD : Real(0));
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const Real rowTerm = cij * diff[p];
-            acc.row[p].add(rowTerm);
-            acc.col[p].add(SYM ? -rowTerm : -(cji * diff[p]));
-            acc.col[p].rotate();
-        }
-    }
-}
-
-// One wavefront per tile.  Skewed schedule: at step k lane l evaluates the
-// pair (row 64 I + l, column 64 J + (l + k) mod 64); the engine stores each
-// tile in that order, so every step is one coalesced 64-value load.  Row
-// partials stay in the lane; column partials and the column locations
-// rotate one lane per step (column partials end in the lane of their
-// column).  Observations are prefetched one group of kGroup steps ahead.
-constexpr int kGroup = 4;
-
+// Single instance: one wavefront per tile of the upper block triangle.
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
     const Real* __restrict__ delta, const Real* __restrict__ x,
     const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
     Real tau, Real halfTau, Real sqrtTau, Real cg, double* __restrict__ tileSums,
     double* __restrict__ part) {
-    const int lane = threadIdx.x % kWave;
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
     if (t >= numTiles) return;
     const uint32_t code = tiles[t];
-    const int I = (int)(code >> 16);
-    const int J = (int)(code & 0xffffu);
-    const bool diag = (I == J);
-
-    const Real* tileStep = delta + (int64_t)t * kTileElems + lane;
-    // Two register buffers of kGroup steps each (ping-pong): while one group
-    // is evaluated the next one is in flight.  The loads of a group are
-    // pinned ahead of the arithmetic with a scheduling barrier so that the
-    // compiler cannot sink them next to their use.
-    Real bufA[kGroup], bufAT[kGroup], bufB[kGroup], bufBT[kGroup];
-    auto load_group = [&](Real (&b)[kGroup], Real (&bT)[kGroup], int k0) {
-#pragma unroll
-        for (int g = 0; g < kGroup; ++g) {
-            const int kn = min(k0 + g, kTile - 1);  // past the end: re-read step 63
-            b[g] = tileStep[kn * kTile];
-            bT[g] = SYM ? b[g] : tileStep[planeStride + kn * kTile];
-        }
-    };
-
-    Real xi[P], xj[P];
-    const Real* xrow = x + ((int64_t)I * kTile + lane) * P;
-    const Real* xcol = x + ((int64_t)J * kTile + lane) * P;  // column (lane + 0) mod 64
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        xi[p] = xrow[p];
-        xj[p] = xcol[p];
-    }
-
-    TileAcc<Real, P> acc;
-    auto run_group = [&](const Real (&b)[kGroup], const Real (&bT)[kGroup], int k0) {
-#pragma unroll
-        for (int g = 0; g < kGroup; ++g) {
-            const int k = k0 + g;
-            // diagonal tiles keep i < j only: lane < (lane + k) mod 64
-            const bool inTri = !diag || (k > 0 && lane + k < kTile);
-            pair_step<Real, P, TRUNC, SYM, GRAD>(acc, xi, xj, b[g], bT[g], inTri, tau, halfTau, sqrtTau, cg);
-#pragma unroll
-            for (int p = 0; p < P; ++p) xj[p] = dpp_mov<kDppWaveRol1, 0xf>(xj[p]);
-        }
-    };
-
-    load_group(bufA, bufAT, 0);
-    for (int k0 = 0; k0 < kTile; k0 += 2 * kGroup) {
-        load_group(bufB, bufBT, k0 + kGroup);
-        __builtin_amdgcn_sched_barrier(0);
-        run_group(bufA, bufAT, k0);
-        load_group(bufA, bufAT, k0 + 2 * kGroup);
-        __builtin_amdgcn_sched_barrier(0);
-        run_group(bufB, bufBT, k0 + kGroup);
-    }
-
-    const double s = wave_sum_to_last(acc.s.value());
-    if (lane == kWave - 1) tileSums[t] = s;
-
-    if (GRAD) {
-        double* rowSlot = part + ((int64_t)I * nb + J) * (P * kTile) + lane;
-        if (diag) {
-#pragma unroll
-            for (int p = 0; p < P; ++p) rowSlot[p * kTile] = acc.row[p].value() + acc.col[p].value();
-        } else {
-            double* colSlot = part + ((int64_t)J * nb + I) * (P * kTile) + lane;
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                rowSlot[p * kTile] = acc.row[p].value();
-                colSlot[p * kTile] = acc.col[p].value();
-            }
-        }
-    }
+    tile_body<Real, P, TRUNC, SYM, GRAD, kGroup>(delta + (int64_t)t * kTileElems, planeStride, x,
+                                                 (int)(code >> 16), (int)(code & 0xffffu), nb, tau, halfTau,
+                                                 sqrtTau, cg, tileSums + t, part);
 }
 
 constexpr int kReduceWaves = 8;

@@ -117,6 +117,22 @@ def chisq(n: int, seed: int = DATA_SEED, missing: bool = False) -> np.ndarray:
     return delta
 
 
+def batch_specs(count: int = 64, n_lo: int = 500, n_hi: int = 8000, dims=(2, 5, 10), seed: int = 999):
+    """(N_k, P_k, tau_k, seed_k) of the uneven batch: N log-uniform in
+    [n_lo, n_hi], P uniform on `dims`, tau uniform in [0.5, 4], dataset seeds
+    1000 + k."""
+    u = uniforms(seed, 1, 3 * count).reshape(count, 3)
+    n = np.rint(np.exp(np.log(n_lo) + u[:, 0] * (np.log(n_hi) - np.log(n_lo)))).astype(int)
+    p = np.asarray(dims)[np.minimum((u[:, 1] * len(dims)).astype(int), len(dims) - 1)]
+    tau = 0.5 + 3.5 * u[:, 2]
+    return [(int(n[k]), int(p[k]), float(tau[k]), 1000 + k) for k in range(count)]
+
+
+def batch_dataset(n: int, p: int, seed: int):
+    """Observations and locations of one batch dataset (euclid10, N(0, I_P))."""
+    return euclid10(n, seed=seed), locations(n, p, seed=seed, key=2)
+
+
 def config(name: str):
     """Inputs of the named workload: (delta, X, precision, truncation)."""
     if name == "n1000_p2":  # reference CPU-runnable case

@@ -1,0 +1,113 @@
+"""Batched persistent launch: every dataset's result must equal, bit for bit,
+the single-instance engine's result on the same dataset (same tile
+arithmetic, same fixed reduction order), whatever launch configuration the
+autotuner picks; and FP64 must match the oracle to 1e-12."""
+import numpy as np
+import pytest
+
+from massivemds_amd import datagen
+from massivemds_amd.batch import MultiDimensionalScalingBatch
+from massivemds_amd.engine import Flags, MdsError, MultiDimensionalScaling
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs():
+    specs = datagen.batch_specs(count=10, n_lo=60, n_hi=900, dims=(1, 2, 3, 5, 10), seed=7)
+    specs[3] = (130, 2, 1.7, 77)  # one asymmetric dataset below
+    return specs
+
+
+def _data(specs):
+    out = []
+    for k, (n, p, tau, seed) in enumerate(specs):
+        delta, x = datagen.batch_dataset(n, p, seed)
+        if k == 3:
+            delta = np.abs(delta + 0.05 * datagen.normals(seed, 9, n * n).reshape(n, n))
+            np.fill_diagonal(delta, 0.0)
+        if k == 5:
+            delta[2, :] = np.nan
+            delta[:, 2] = np.nan
+        out.append((delta, x, tau))
+    return out
+
+
+def _batch(specs, data, flags):
+    b = MultiDimensionalScalingBatch([s[1] for s in specs], [s[0] for s in specs], flags, 0)
+    for k, (delta, x, tau) in enumerate(data):
+        b.setPairwiseData(k, delta)
+        b.updateLocations(k, x)
+        b.setParameters(k, [tau])
+    return b
+
+
+def _single(delta, x, tau, flags):
+    n, p = x.shape
+    e = MultiDimensionalScaling(p, n, flags, 0)
+    e.setPairwiseData(delta)
+    e.updateLocations(-1, x)
+    e.setParameters([tau])
+    s = e.getSumOfIncrements()
+    g = e.getLogLikelihoodGradient().reshape(n, p)
+    e.close()
+    return s, g
+
+
+@pytest.mark.parametrize("fp32", [True, False])
+def test_batch_equals_single_instance(oracle, fp32):
+    flags = Flags.OPENCL | Flags.LEFT_TRUNCATION | (Flags.FLOAT if fp32 else 0)
+    specs = _specs()
+    data = _data(specs)
+    b = _batch(specs, data, flags)
+    b.evaluate()
+    sums = b.sums()
+    for k, (delta, x, tau) in enumerate(data):
+        s, g = _single(delta, x, tau, flags)
+        assert sums[k] == s, (k, sums[k], s)
+        np.testing.assert_array_equal(b.gradient(k), g)
+        if not fp32:
+            s_ref = oracle.sum_of_increments(delta, x, tau, True)
+            g_ref = oracle.gradient(delta, x, tau, True)
+            assert abs(s - s_ref) <= 1e-12 * abs(s_ref)
+            assert np.max(np.abs(g - g_ref)) <= 1e-12 * np.max(np.abs(g_ref))
+    b.close()
+
+
+def test_schedule_independent_bitwise(tmp_path):
+    flags = Flags.OPENCL | Flags.LEFT_TRUNCATION | Flags.FLOAT
+    specs = _specs()
+    data = _data(specs)
+    b = _batch(specs, data, flags)
+    b.evaluate()
+    ref_s = b.sums()
+    ref_g = [b.gradient(k) for k in range(len(specs))]
+    for cfg in [(1, 1, 1, 0), (8, 4, 2, 0), (2, 8, 4, 1), (4, 2, 1, 1)]:
+        b.set_config(*cfg)
+        b.evaluate()
+        np.testing.assert_array_equal(b.sums(), ref_s)
+        for k in range(len(specs)):
+            np.testing.assert_array_equal(b.gradient(k), ref_g[k])
+    cache = tmp_path / "tune.txt"
+    ms = b.autotune(str(cache))
+    assert ms > 0 and cache.exists()
+    chosen = b.get_config()
+    b.set_config(1, 1, 1, 0)
+    assert b.autotune(str(cache)) == pytest.approx(ms)  # read back from the cache
+    assert b.get_config() == chosen
+    b.evaluate()
+    np.testing.assert_array_equal(b.sums(), ref_s)
+    b.close()
+
+
+def test_batch_errors():
+    with pytest.raises(MdsError):
+        MultiDimensionalScalingBatch([2, 11], [50, 60], Flags.OPENCL)
+    b = MultiDimensionalScalingBatch([2, 3], [50, 60], Flags.OPENCL)
+    with pytest.raises(MdsError) as info:
+        b.evaluate()
+    assert info.value.status == 3
+    with pytest.raises(MdsError):
+        b.setPairwiseData(2, np.zeros((50, 50)))
+    with pytest.raises(MdsError):
+        b.updateLocations(0, np.zeros(10))
+    b.close()
