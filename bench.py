@@ -177,6 +177,28 @@ def _bench_batch(steps, warmup, count=64, tune_cache=None):
     print(json.dumps(out))
 
 
+def _bench_hmc(iterations=2000, burn_in=500, n=5000, p=2):
+    """Plain HMC on the GPU: N = 5000, P = 2, FP64, euclid10 with 2% missing,
+    truncation, 20 leapfrog steps over trajectory 0.1, tau and the trait
+    precision learned, one hipGraph per leapfrog step."""
+    from massivemds_amd import datagen, hmc
+
+    delta = datagen.euclid10(n, missing=0.02)
+    res = hmc.hmcsampler(iterations, burnIn=burn_in, data=delta, latentDimension=p, learnPrec=True,
+                         learnTraitPrec=True, trajectory=0.1, mdsPrecision=1.0, truncation=True, seed=666)
+    kept = iterations - burn_in
+    out = {
+        "workload": f"hmc_n{n}_p{p}_fp64_trunc_missing2", "iterations": iterations, "burn_in": burn_in,
+        "accept_rate": res["acprat"], "precision_accept_rate": res["precision_acceptance"],
+        "sampling_seconds": res["Time"], "ms_per_iteration": res["Time"] / kept * 1e3,
+        "min_ess": res["min_ess"], "min_ess_per_s": res["min_ess_per_s"],
+        "ess": {k: round(v, 1) for k, v in res["ess"].items()},
+        "posterior_mean_precision": float(np.mean(res["precision"])),
+        "evaluations": res["evaluations"], "total_seconds": res["total_seconds"],
+    }
+    print(json.dumps(out))
+
+
 def _calibrate_copy(mib):
     """A plain elementwise copy kernel of known size (torch, as plumbing)."""
     import torch
@@ -198,8 +220,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--workload", default="n10000", choices=["n10000", "batch64"],
-                    help="n10000: the metric's workload (default); batch64: the uneven batch")
+    ap.add_argument("--workload", default="n10000", choices=["n10000", "batch64", "hmc5000"],
+                    help="n10000: the metric's workload (default); batch64: the uneven batch; "
+                         "hmc5000: plain HMC")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch64)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to scale "
@@ -210,6 +233,9 @@ def main():
         return
     if args.workload == "batch64":
         _bench_batch(args.steps, args.warmup, tune_cache=args.tune_cache)
+        return
+    if args.workload == "hmc5000":
+        _bench_hmc()
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a "

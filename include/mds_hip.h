@@ -159,6 +159,56 @@ int mds_hip_batch_get_config(mds_hip_batch_handle h, int* wavesPerBlock, int* bl
 int mds_hip_batch_set_timing(mds_hip_batch_handle h, int enable);
 int mds_hip_batch_get_timing(mds_hip_batch_handle h, double* ms, long long* count);
 
+/*
+ * Device-resident HMC sampler: the GPU counterpart of the reference's R
+ * hmcsampler (R/Untitled.R:477-729) for the model without a tree (row
+ * covariance I_N): potential = -(matrix-normal log prior + MDS
+ * log-likelihood), leapfrog with step trajectory / leapfrogSteps, one
+ * hipGraph per leapfrog step, Metropolis test in FP64 on the host.
+ * learnPrecision: random-walk Metropolis on tau with the reference proposal
+ * |U(tau - w, tau + w)| under truncation, Gamma Gibbs draw otherwise.
+ * learnTraitPrecision: Wishart(P + N, (I + X^T X)^-1) Gibbs draw every
+ * traitInvWeight iterations.  The engine must hold data and a precision; on
+ * return it holds the final locations and precision.
+ *
+ * trace: (iterations - burnIn) rows of MDS_HIP_HMC_TRACE_COLUMNS values --
+ * log posterior, tau, trace(T), five pairwise distances |x_a - x_b| for
+ * (a, b) = (0, 1), (0, N/2), (N/3, 2N/3), (1, N-1), (N/4, 3N/4), and the
+ * norms |x_0|, |x_{N/2}|, |x_{N-1}| (functionals invariant to rotations and
+ * reflections of the configuration).
+ */
+#define MDS_HIP_HMC_TRACE_COLUMNS 11
+
+typedef struct {
+    int iterations;
+    int burnIn;
+    int leapfrogSteps;
+    double trajectory;
+    int learnPrecision;
+    int learnTraitPrecision;
+    int traitInvWeight;
+    double precisionProposalWidth;
+    double priorDf;              /* <= 0: P */
+    unsigned long long seed;
+    int useGraph;
+} mds_hip_hmc_options;
+
+typedef struct {
+    double acceptRate;
+    double precisionAcceptRate;
+    double samplingSeconds;      /* iterations after burn-in */
+    double totalSeconds;
+    double evaluations;          /* likelihood (+ gradient) passes */
+    double finalPrecision;
+    double stepSize;
+    double finalPotential;
+} mds_hip_hmc_stats;
+
+int mds_hip_hmc_default_options(mds_hip_hmc_options* options);
+int mds_hip_hmc_run(mds_hip_handle h, const mds_hip_hmc_options* options, const double* initialLocations,
+                    size_t length, double* trace, size_t traceLength, double* finalLocations,
+                    mds_hip_hmc_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,368 +1,10 @@
-// libmds_hip.so: the HIP engine behind the C ABI declared in include/mds_hip.h.
-//
-// State and caching follow the reference CPU engine
-// (src/NewMultiDimensionalScaling.hpp:134-302):
-//   * updateLocations invalidates the cached sum of increments (:165);
-//   * setParameters invalidates it only with truncation (:262-274), and the
-//     non-truncated sum is returned as 0.5 * precision * cached (:198-202);
-//   * storeState copies the locations, sum and precision (:205-217);
-//     restoreState restores sum and precision, marks the sum known and swaps
-//     the location buffers (:234-255);
-//   * setPairwiseData does not touch the cached sum (:257-260);
-//   * every evaluation is a full O(N^2) pass (the reference's incremental
-//     path is dead code, :563-566).
-// In addition the engine keeps the device gradient of the last fused
-// evaluation and reuses it while the state is unchanged (the reference
-// recomputes it; the values are identical), and a gradient call fills the
-// sum cache when the reference would have recomputed the same value.
+// libmds_hip.so: the C ABI declared in include/mds_hip.h over the engine in
+// mds_engine.hpp.
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
-#include <atomic>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <memory>
-#include <stdexcept>
 #include <string>
-#include <thread>
-#include <vector>
 
-#include "mds_hip.h"
-#include "mds_host.hpp"
-#include "mds_internal.hpp"
-
-namespace mdship {
-
-struct Engine {
-    int P = 0, N = 0, kdim = 0, device = 0;
-    long flags = 0;
-    bool fp32 = false, truncated = false;
-    int nb = 0, npad = 0, numTiles = 0;
-    size_t realSize = 8;
-
-    hipStream_t stream = nullptr;
-    void* dDelta = nullptr;
-    size_t deltaBytes = 0;
-    bool symmetric = true;
-    bool dataSet = false;
-    void* dX[2] = {nullptr, nullptr};
-    int cur = 0;
-    uint32_t* dTiles = nullptr;
-    double* dTileSums = nullptr;
-    double* dPart = nullptr;
-    double* dGrad = nullptr;
-    double* dSum = nullptr;
-    double* hGrad = nullptr;   // pinned
-    double* hSum = nullptr;    // pinned
-    void* hX = nullptr;        // pinned staging for location uploads
-
-    bool precisionSet = false;
-    double precision = 0.0, storedPrecision = 0.0;
-    double sumOfIncrements = 0.0, storedSumOfIncrements = 0.0;
-    bool sumKnown = false;
-    bool devSumFresh = false;  // dSum matches the current state
-    bool gradKnown = false;    // dGrad matches the current state
-
-    bool timing = false;
-    std::vector<hipEvent_t> eventPool;
-    std::vector<hipEvent_t> pending;  // triples: start, after tiles, after reduce
-    double tileMsTotal = 0.0, evalMsTotal = 0.0;
-    long long timedCount = 0;
-
-    std::string lastError;
-
-    Engine(int p, int n, long f, int dev) : P(p), N(n), flags(f) {
-        if (p < 1 || p > kMaxDim)
-            throw Error(MDS_HIP_ERR_ARGUMENT,
-                        "embedding dimension must be in [1, " + std::to_string(kMaxDim) + "]");
-        if (n < 1) throw Error(MDS_HIP_ERR_ARGUMENT, "location count must be >= 1");
-        int count = 0;
-        check(hipGetDeviceCount(&count), "hipGetDeviceCount");
-        if (count <= 0) throw Error(MDS_HIP_ERR_DEVICE, "no HIP device");
-        device = (dev < 0 || dev >= count) ? 0 : dev;
-        check(hipSetDevice(device), "hipSetDevice");
-        fp32 = (flags & MDS_HIP_FLAG_FLOAT) != 0;
-        truncated = (flags & MDS_HIP_FLAG_LEFT_TRUNCATION) != 0;
-        realSize = fp32 ? sizeof(float) : sizeof(double);
-        kdim = kernel_dim(P);
-        nb = (N + kTile - 1) / kTile;
-        if (nb > 0xffff) throw Error(MDS_HIP_ERR_ARGUMENT, "location count too large");
-        npad = nb * kTile;
-        const std::vector<uint32_t> tiles = make_tiles(nb);
-        numTiles = (int)tiles.size();
-        try {
-            check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-            const size_t xBytes = (size_t)npad * kdim * realSize;
-            for (int k = 0; k < 2; ++k) {
-                check(hipMalloc(&dX[k], xBytes), "hipMalloc(locations)");
-                check(hipMemsetAsync(dX[k], 0, xBytes, stream), "hipMemset(locations)");
-            }
-            check(hipMalloc((void**)&dTiles, tiles.size() * sizeof(uint32_t)), "hipMalloc(tiles)");
-            check(hipMemcpyAsync(dTiles, tiles.data(), tiles.size() * sizeof(uint32_t),
-                                 hipMemcpyHostToDevice, stream),
-                  "hipMemcpy(tiles)");
-            check(hipMalloc((void**)&dTileSums, tiles.size() * sizeof(double)), "hipMalloc(sums)");
-            check(hipMalloc((void**)&dPart, (size_t)nb * nb * kdim * kTile * sizeof(double)),
-                  "hipMalloc(partials)");
-            check(hipMalloc((void**)&dGrad, (size_t)N * P * sizeof(double)), "hipMalloc(gradient)");
-            check(hipMalloc((void**)&dSum, sizeof(double)), "hipMalloc(sum)");
-            check(hipHostMalloc((void**)&hGrad, (size_t)N * P * sizeof(double), hipHostMallocDefault),
-                  "hipHostMalloc(gradient)");
-            check(hipHostMalloc((void**)&hSum, sizeof(double), hipHostMallocDefault), "hipHostMalloc(sum)");
-            check(hipHostMalloc(&hX, (size_t)npad * kdim * realSize, hipHostMallocDefault),
-                  "hipHostMalloc(locations)");
-            std::memset(hX, 0, (size_t)npad * kdim * realSize);
-            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        } catch (...) {
-            release();
-            throw;
-        }
-    }
-
-    ~Engine() { release(); }
-
-    void release() {
-        (void)hipSetDevice(device);
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
-        for (hipEvent_t e : pending) (void)hipEventDestroy(e);
-        eventPool.clear();
-        pending.clear();
-        (void)hipFree(dDelta);
-        (void)hipFree(dX[0]);
-        (void)hipFree(dX[1]);
-        (void)hipFree(dTiles);
-        (void)hipFree(dTileSums);
-        (void)hipFree(dPart);
-        (void)hipFree(dGrad);
-        (void)hipFree(dSum);
-        (void)hipHostFree(hGrad);
-        (void)hipHostFree(hSum);
-        (void)hipHostFree(hX);
-        if (stream) (void)hipStreamDestroy(stream);
-        dDelta = dX[0] = dX[1] = nullptr;
-        dTiles = nullptr;
-        dTileSums = dPart = dGrad = dSum = hGrad = hSum = nullptr;
-        hX = nullptr;
-        stream = nullptr;
-    }
-
-    void invalidate_device_results() {
-        devSumFresh = false;
-        gradKnown = false;
-    }
-
-    // src/NewMultiDimensionalScaling.hpp:134-166
-    void updateLocations(int index, const double* loc, size_t length) {
-        if (!loc) throw Error(MDS_HIP_ERR_ARGUMENT, "null location pointer");
-        size_t rows, row0;
-        if (index == -1) {
-            if (length != (size_t)N * P)
-                throw Error(MDS_HIP_ERR_ARGUMENT, "updateLocations(-1): length must be N*P");
-            rows = N;
-            row0 = 0;
-        } else {
-            if (index < 0 || index >= N)
-                throw Error(MDS_HIP_ERR_ARGUMENT, "updateLocations: index out of range");
-            if (length != (size_t)P)
-                throw Error(MDS_HIP_ERR_ARGUMENT, "updateLocations(i): length must be P");
-            rows = 1;
-            row0 = index;
-        }
-        // Stage as [rows][kdim] in the engine's real type, padding with zeros.
-        if (fp32) {
-            float* s = (float*)hX;
-            for (size_t r = 0; r < rows; ++r)
-                for (int k = 0; k < kdim; ++k) s[r * kdim + k] = k < P ? (float)loc[r * P + k] : 0.0f;
-        } else {
-            double* s = (double*)hX;
-            for (size_t r = 0; r < rows; ++r)
-                for (int k = 0; k < kdim; ++k) s[r * kdim + k] = k < P ? loc[r * P + k] : 0.0;
-        }
-        char* dst = (char*)dX[cur] + row0 * kdim * realSize;
-        check(hipMemcpyAsync(dst, hX, rows * kdim * realSize, hipMemcpyHostToDevice, stream),
-              "hipMemcpy(locations)");
-        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        sumKnown = false;
-        invalidate_device_results();
-    }
-
-    void setPairwiseData(const double* data, size_t length) {
-        if (!data) throw Error(MDS_HIP_ERR_ARGUMENT, "null data pointer");
-        if (length != (size_t)N * N)
-            throw Error(MDS_HIP_ERR_ARGUMENT, "setPairwiseData: length must be N*N");
-        const std::vector<uint32_t> tiles = make_tiles(nb);
-        const size_t planeElems = (size_t)numTiles * kTileElems;
-        std::unique_ptr<char[]> host(new char[2 * planeElems * realSize]);
-        bool sym;
-        if (fp32)
-            sym = pack_tiles<float>(data, N, tiles, (float*)host.get(),
-                                    (float*)host.get() + planeElems);
-        else
-            sym = pack_tiles<double>(data, N, tiles, (double*)host.get(),
-                                     (double*)host.get() + planeElems);
-        const size_t bytes = (sym ? 1 : 2) * planeElems * realSize;
-        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        if (bytes != deltaBytes) {
-            check(hipFree(dDelta), "hipFree(observations)");
-            dDelta = nullptr;
-            deltaBytes = 0;
-            check(hipMalloc(&dDelta, bytes), "hipMalloc(observations)");
-            deltaBytes = bytes;
-        }
-        check(hipMemcpy(dDelta, host.get(), bytes, hipMemcpyHostToDevice), "hipMemcpy(observations)");
-        symmetric = sym;
-        dataSet = true;
-        invalidate_device_results();
-    }
-
-    // src/NewMultiDimensionalScaling.hpp:262-274
-    void setParameters(const double* data, size_t length) {
-        if (!data || length != 1)
-            throw Error(MDS_HIP_ERR_ARGUMENT, "setParameters: expects exactly the precision");
-        if (!std::isfinite(data[0]) || data[0] <= 0.0)
-            throw Error(MDS_HIP_ERR_ARGUMENT, "setParameters: precision must be finite and > 0");
-        precision = data[0];
-        precisionSet = true;
-        if (truncated) sumKnown = false;
-        invalidate_device_results();
-    }
-
-    void makeDirty() {  // :276-279
-        sumKnown = false;
-        invalidate_device_results();
-    }
-
-    void storeState() {  // :205-217
-        storedSumOfIncrements = sumOfIncrements;
-        check(hipMemcpyAsync(dX[1 - cur], dX[cur], (size_t)npad * kdim * realSize,
-                             hipMemcpyDeviceToDevice, stream),
-              "hipMemcpy(store)");
-        storedPrecision = precision;
-    }
-
-    void restoreState() {  // :234-255
-        sumOfIncrements = storedSumOfIncrements;
-        sumKnown = true;
-        precision = storedPrecision;
-        cur = 1 - cur;
-        invalidate_device_results();
-    }
-
-    hipEvent_t take_event() {
-        if (eventPool.empty()) {
-            hipEvent_t e;
-            check(hipEventCreate(&e), "hipEventCreate");
-            return e;
-        }
-        hipEvent_t e = eventPool.back();
-        eventPool.pop_back();
-        return e;
-    }
-
-    void evaluate(bool gradient) {
-        if (!dataSet) throw Error(MDS_HIP_ERR_STATE, "pairwise data not set");
-        if (!precisionSet) throw Error(MDS_HIP_ERR_STATE, "precision not set");
-        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
-        if (timing) {
-            e0 = take_event();
-            e1 = take_event();
-            e2 = take_event();
-            check(hipEventRecord(e0, stream), "hipEventRecord");
-        }
-        TileLaunch a;
-        a.delta = dDelta;
-        a.x = dX[cur];
-        a.tiles = dTiles;
-        a.numTiles = numTiles;
-        a.nb = nb;
-        a.kdim = kdim;
-        a.planeStride = (int64_t)numTiles * kTileElems;
-        a.tau = precision;
-        a.truncated = truncated;
-        a.symmetric = symmetric;
-        a.gradient = gradient;
-        a.tileSums = dTileSums;
-        a.part = dPart;
-        check(fp32 ? launch_tiles_f32(a, stream) : launch_tiles_f64(a, stream), "tile kernel");
-        if (timing) check(hipEventRecord(e1, stream), "hipEventRecord");
-        ReduceLaunch r;
-        r.part = dPart;
-        r.tileSums = dTileSums;
-        r.numTiles = numTiles;
-        r.nb = nb;
-        r.kdim = kdim;
-        r.n = N;
-        r.p = P;
-        r.gradient = gradient;
-        r.grad = dGrad;
-        r.sumOut = dSum;
-        check(launch_reduce(r, stream), "reduce kernel");
-        if (timing) {
-            check(hipEventRecord(e2, stream), "hipEventRecord");
-            pending.push_back(e0);
-            pending.push_back(e1);
-            pending.push_back(e2);
-        }
-        devSumFresh = true;
-        if (gradient) gradKnown = true;
-    }
-
-    double getSumOfIncrements() {  // :193-203
-        if (!sumKnown) {
-            if (!devSumFresh) evaluate(false);
-            check(hipMemcpyAsync(hSum, dSum, sizeof(double), hipMemcpyDeviceToHost, stream),
-                  "hipMemcpy(sum)");
-            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-            sumOfIncrements = *hSum;
-            sumKnown = true;
-        }
-        return truncated ? sumOfIncrements : 0.5 * precision * sumOfIncrements;
-    }
-
-    void getLogLikelihoodGradient(double* result, size_t length) {  // :281-302
-        if (!result) throw Error(MDS_HIP_ERR_ARGUMENT, "null gradient pointer");
-        if (length != (size_t)N * P)
-            throw Error(MDS_HIP_ERR_ARGUMENT, "getLogLikelihoodGradient: length must be N*P");
-        if (!gradKnown) evaluate(true);
-        const bool fillSum = !sumKnown;
-        check(hipMemcpyAsync(hGrad, dGrad, length * sizeof(double), hipMemcpyDeviceToHost, stream),
-              "hipMemcpy(gradient)");
-        if (fillSum)
-            check(hipMemcpyAsync(hSum, dSum, sizeof(double), hipMemcpyDeviceToHost, stream),
-                  "hipMemcpy(sum)");
-        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        std::memcpy(result, hGrad, length * sizeof(double));
-        if (fillSum) {
-            sumOfIncrements = *hSum;
-            sumKnown = true;
-        }
-    }
-
-    void getTiming(double* tileMs, double* evalMs, long long* count) {
-        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        for (size_t k = 0; k + 2 < pending.size(); k += 3) {
-            float a = 0.f, b = 0.f;
-            check(hipEventElapsedTime(&a, pending[k], pending[k + 1]), "hipEventElapsedTime");
-            check(hipEventElapsedTime(&b, pending[k], pending[k + 2]), "hipEventElapsedTime");
-            tileMsTotal += a;
-            evalMsTotal += b;
-            ++timedCount;
-        }
-        for (hipEvent_t e : pending) eventPool.push_back(e);
-        pending.clear();
-        if (tileMs) *tileMs = tileMsTotal;
-        if (evalMs) *evalMs = evalMsTotal;
-        if (count) *count = timedCount;
-    }
-};
-
-}  // namespace mdship
-
-struct mds_hip_engine {
-    mdship::Engine* e;
-};
+#include "mds_engine.hpp"
 
 static thread_local std::string g_create_error = "no error";
 

@@ -1,0 +1,564 @@
+// Device-resident Hamiltonian Monte Carlo for Bayesian MDS, the GPU
+// counterpart of the reference's R sampler hmcsampler (R/Untitled.R:477-729)
+// with Potential (:427-445) and the matrix-normal prior dmatrixnorm (:68-95)
+// for the case without a tree (row covariance I_N):
+//
+//   potential U(X) = -(log prior(X | T) + log-likelihood(X, tau)),
+//   log prior = -tr(T X^T X) / 2 - (N P / 2) log 2 pi + (N / 2) log det T,
+//   log-likelihood = (log tau - log 2 pi) N (N - 1) / 4 - sumOfIncrements
+//   (R/mds.R:24-28), momentum ~ N(0, I), leapfrog with step trajectory / L.
+//
+// Locations, momenta and gradients stay on the device; one leapfrog step is
+// captured once as a hipGraph (kick + drift, the fused likelihood/gradient
+// kernels, kick) and replayed L times per iteration.  The precision and the
+// trait precision live in a device parameter block, so the graph stays valid
+// when they are updated.  The Hamiltonian and the Metropolis log-ratio are
+// formed on the host in FP64.
+//
+// Between HMC moves:
+//   * tau: with truncation, random-walk Metropolis with the reference's
+//     proposal |U(tau - w, tau + w)| (R/Untitled.R:668-691), evaluated at the
+//     current (accepted) locations; without truncation the flat-prior
+//     posterior of tau is Gamma(1 + N(N-1)/4, C / 2) with C the unscaled
+//     sum of squared residuals, and tau is drawn from it (Gibbs);
+//   * T: Gibbs draw from Wishart(d0 + N, (T0 + X^T X)^-1) (R/Untitled.R:694-700),
+//     d0 = P, T0 = I_P.
+// Unlike the reference the current potential is recomputed after every tau
+// or T update, and the tau step uses the current locations after a
+// rejection.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mds_datagen.hpp"
+#include "mds_device.hpp"
+#include "mds_engine.hpp"
+#include "mds_hmc.hpp"
+
+namespace mdship {
+
+// ------------------------------------------------------------------ kernels
+
+// splitmix64 finaliser, as mdsgen::mix on the host.
+__device__ __forceinline__ uint64_t dmix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Standard normal momenta from a counter-based stream keyed by (seed,
+// iteration): the same splitmix64 / Box-Muller construction as the host
+// generator, one pair per thread.
+__global__ void momentum_kernel(double* __restrict__ mom, int count, uint64_t seed, uint64_t iteration) {
+    const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = 2 * pair;
+    if (k >= count) return;
+    const uint64_t state = dmix(seed * 0x100000001B3ULL + (iteration + 1) * 0xC2B2AE3D27D4EB4FULL);
+    const uint64_t z1 = dmix(state + (uint64_t)(k + 1) * 0x9E3779B97F4A7C15ULL);
+    const uint64_t z2 = dmix(state + (uint64_t)(k + 2) * 0x9E3779B97F4A7C15ULL);
+    const double u1 = ((double)(z1 >> 11) + 0.5) * 0x1.0p-53;
+    const double u2 = ((double)(z2 >> 11) + 0.5) * 0x1.0p-53;
+    const double r = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    mom[k] = r * c;
+    if (k + 1 < count) mom[k + 1] = r * s;
+}
+
+// Row i: g = grad log-likelihood + grad log prior = grad_ll - x_i T.
+// drift: mom += halfEps * g(x); x += eps * mom.  kick: mom += halfEps * g(x).
+template <typename Real, bool DRIFT>
+__global__ void kick_kernel(Real* __restrict__ x, double* __restrict__ mom, const double* __restrict__ grad,
+                            const double* __restrict__ params, int n, int p, int kdim) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double eps = params[kHmcEps];
+    const double halfEps = 0.5 * eps;
+    const double* T = params + kHmcTrait;
+    Real* xi = x + (int64_t)i * kdim;
+    for (int d = 0; d < p; ++d) {
+        double prior = 0.0;
+        for (int e = 0; e < p; ++e) prior += (double)xi[e] * T[e * p + d];
+        const double g = grad[(int64_t)i * p + d] - prior;
+        mom[(int64_t)i * p + d] += halfEps * g;
+    }
+    if (DRIFT) {
+        for (int d = 0; d < p; ++d) xi[d] = (Real)((double)xi[d] + eps * mom[(int64_t)i * p + d]);
+    }
+}
+
+// out[0] = sum mom^2 / 2, out[1] = sum_i x_i^T T x_i (fixed-order reductions).
+constexpr int kEnergyThreads = 512;
+
+template <typename Real>
+__global__ __launch_bounds__(kEnergyThreads) void energy_kernel(const Real* __restrict__ x,
+                                                                const double* __restrict__ mom,
+                                                                const double* __restrict__ params, int n, int p,
+                                                                int kdim, double* __restrict__ out) {
+    __shared__ double red[2][kEnergyThreads];
+    const double* T = params + kHmcTrait;
+    double k = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < n; i += kEnergyThreads) {
+        const Real* xi = x + (int64_t)i * kdim;
+        for (int d = 0; d < p; ++d) {
+            const double m = mom[(int64_t)i * p + d];
+            k += 0.5 * m * m;
+            double row = 0.0;
+            for (int e = 0; e < p; ++e) row += (double)xi[e] * T[e * p + d];
+            q += row * (double)xi[d];
+        }
+    }
+    red[0][threadIdx.x] = k;
+    red[1][threadIdx.x] = q;
+    __syncthreads();
+    for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+    }
+}
+
+// ------------------------------------------------------------------ host
+
+// Marsaglia & Tsang (2000), "A simple method for generating gamma variables".
+double gamma_draw(mdsgen::Stream& s, double shape) {
+    if (shape < 1.0) {
+        const double u = s.uniform();
+        return gamma_draw(s, shape + 1.0) * std::pow(u, 1.0 / shape);
+    }
+    const double d = shape - 1.0 / 3.0, c = 1.0 / std::sqrt(9.0 * d);
+    for (;;) {
+        double z, v;
+        do {
+            const double u1 = s.uniform(), u2 = s.uniform();
+            z = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+            v = 1.0 + c * z;
+        } while (v <= 0.0);
+        v = v * v * v;
+        const double u = s.uniform();
+        if (std::log(u) < 0.5 * z * z + d - d * v + d * std::log(v)) return d * v;
+    }
+}
+
+// Cholesky factor (lower) of a symmetric positive-definite p x p matrix.
+static bool cholesky(const std::vector<double>& a, int p, std::vector<double>& l) {
+    l.assign((size_t)p * p, 0.0);
+    for (int i = 0; i < p; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = a[i * p + j];
+            for (int k = 0; k < j; ++k) s -= l[i * p + k] * l[j * p + k];
+            if (i == j) {
+                if (s <= 0.0) return false;
+                l[i * p + i] = std::sqrt(s);
+            } else {
+                l[i * p + j] = s / l[j * p + j];
+            }
+        }
+    return true;
+}
+
+static std::vector<double> spd_inverse(const std::vector<double>& a, int p) {
+    std::vector<double> l;
+    if (!cholesky(a, p, l)) throw Error(MDS_HIP_ERR_STATE, "matrix not positive definite");
+    // inverse of L, then (L^-1)^T L^-1
+    std::vector<double> li((size_t)p * p, 0.0);
+    for (int j = 0; j < p; ++j) {
+        li[j * p + j] = 1.0 / l[j * p + j];
+        for (int i = j + 1; i < p; ++i) {
+            double s = 0.0;
+            for (int k = j; k < i; ++k) s -= l[i * p + k] * li[k * p + j];
+            li[i * p + j] = s / l[i * p + i];
+        }
+    }
+    std::vector<double> inv((size_t)p * p, 0.0);
+    for (int i = 0; i < p; ++i)
+        for (int j = 0; j < p; ++j) {
+            double s = 0.0;
+            for (int k = std::max(i, j); k < p; ++k) s += li[k * p + i] * li[k * p + j];
+            inv[i * p + j] = s;
+        }
+    return inv;
+}
+
+static double log_det_spd(const std::vector<double>& a, int p) {
+    std::vector<double> l;
+    if (!cholesky(a, p, l)) throw Error(MDS_HIP_ERR_STATE, "matrix not positive definite");
+    double s = 0.0;
+    for (int i = 0; i < p; ++i) s += 2.0 * std::log(l[i * p + i]);
+    return s;
+}
+
+// Wishart(df, scale) by the Bartlett decomposition: with scale = L L^T and
+// A lower triangular, A_ii^2 ~ chi^2(df - i), A_ij ~ N(0, 1) below the
+// diagonal, W = L A A^T L^T.
+std::vector<double> wishart_draw(mdsgen::Stream& s, double df, const std::vector<double>& scale, int p) {
+    std::vector<double> l;
+    if (!cholesky(scale, p, l)) throw Error(MDS_HIP_ERR_STATE, "Wishart scale not positive definite");
+    std::vector<double> A((size_t)p * p, 0.0);
+    for (int i = 0; i < p; ++i) {
+        A[i * p + i] = std::sqrt(2.0 * gamma_draw(s, 0.5 * (df - i)));
+        for (int j = 0; j < i; ++j) {
+            const double u1 = s.uniform(), u2 = s.uniform();
+            A[i * p + j] = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+        }
+    }
+    std::vector<double> la((size_t)p * p, 0.0);  // L A
+    for (int i = 0; i < p; ++i)
+        for (int j = 0; j < p; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < p; ++k) v += l[i * p + k] * A[k * p + j];
+            la[i * p + j] = v;
+        }
+    std::vector<double> w((size_t)p * p, 0.0);
+    for (int i = 0; i < p; ++i)
+        for (int j = 0; j < p; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < p; ++k) v += la[i * p + k] * la[j * p + k];
+            w[i * p + j] = v;
+        }
+    return w;
+}
+
+Hmc::Hmc(Engine& engine, const mds_hip_hmc_options& options) : e(engine), opt(options) {
+    if (!e.dataSet) throw Error(MDS_HIP_ERR_STATE, "pairwise data not set");
+    if (!e.precisionSet) throw Error(MDS_HIP_ERR_STATE, "precision not set");
+    if (opt.iterations < 1 || opt.leapfrogSteps < 1 || opt.burnIn < 0 || opt.burnIn >= opt.iterations)
+        throw Error(MDS_HIP_ERR_ARGUMENT, "invalid iteration counts");
+    if (!(opt.trajectory > 0.0)) throw Error(MDS_HIP_ERR_ARGUMENT, "trajectory must be > 0");
+    if (opt.traitInvWeight < 1) throw Error(MDS_HIP_ERR_ARGUMENT, "traitInvWeight must be >= 1");
+    n = e.N;
+    p = e.P;
+    const size_t np = (size_t)n * p;
+    const size_t xBytes = (size_t)e.npad * e.kdim * e.realSize;
+    check(hipMalloc((void**)&dMom, np * sizeof(double)), "hipMalloc(momentum)");
+    check(hipMalloc((void**)&dGradSave, np * sizeof(double)), "hipMalloc(gradient copy)");
+    check(hipMalloc(&dXSave, xBytes), "hipMalloc(location copy)");
+    check(hipMalloc((void**)&dParams, kHmcParamCount * sizeof(double)), "hipMalloc(parameters)");
+    check(hipMalloc((void**)&dEnergy, 2 * sizeof(double)), "hipMalloc(energy)");
+    check(hipHostMalloc((void**)&hParams, kHmcParamCount * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+    check(hipHostMalloc((void**)&hScalars, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+    check(hipHostMalloc(&hX, xBytes, hipHostMallocDefault), "hipHostMalloc");
+    std::memset(hParams, 0, kHmcParamCount * sizeof(double));
+    tau = e.precision;
+    trait.assign((size_t)p * p, 0.0);
+    for (int d = 0; d < p; ++d) trait[d * p + d] = 1.0;  // solve(traitVcv) with traitVcv = I_P
+    eps = opt.trajectory / opt.leapfrogSteps;
+}
+
+Hmc::~Hmc() {
+    (void)hipStreamSynchronize(e.stream);
+    if (graphExec) (void)hipGraphExecDestroy(graphExec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipFree(dMom);
+    (void)hipFree(dGradSave);
+    (void)hipFree(dXSave);
+    (void)hipFree(dParams);
+    (void)hipFree(dEnergy);
+    (void)hipHostFree(hParams);
+    (void)hipHostFree(hScalars);
+    (void)hipHostFree(hX);
+}
+
+void Hmc::uploadParams() {
+    hParams[kHmcTau] = tau;
+    hParams[kHmcEps] = eps;
+    for (int k = 0; k < p * p; ++k) hParams[kHmcTrait + k] = trait[k];
+    check(hipMemcpyAsync(dParams, hParams, kHmcParamCount * sizeof(double), hipMemcpyHostToDevice, e.stream),
+          "hipMemcpy(parameters)");
+}
+
+void Hmc::leapfrogStep() {
+    const int threads = 256, blocks = (n + threads - 1) / threads;
+    if (e.fp32) {
+        hipLaunchKernelGGL((kick_kernel<float, true>), dim3(blocks), dim3(threads), 0, e.stream, (float*)e.dX[e.cur],
+                           dMom, e.dGrad, dParams, n, p, e.kdim);
+    } else {
+        hipLaunchKernelGGL((kick_kernel<double, true>), dim3(blocks), dim3(threads), 0, e.stream,
+                           (double*)e.dX[e.cur], dMom, e.dGrad, dParams, n, p, e.kdim);
+    }
+    check(hipGetLastError(), "kick/drift kernel");
+    gradientAtCurrent();
+    if (e.fp32) {
+        hipLaunchKernelGGL((kick_kernel<float, false>), dim3(blocks), dim3(threads), 0, e.stream,
+                           (float*)e.dX[e.cur], dMom, e.dGrad, dParams, n, p, e.kdim);
+    } else {
+        hipLaunchKernelGGL((kick_kernel<double, false>), dim3(blocks), dim3(threads), 0, e.stream,
+                           (double*)e.dX[e.cur], dMom, e.dGrad, dParams, n, p, e.kdim);
+    }
+    check(hipGetLastError(), "kick kernel");
+}
+
+void Hmc::gradientAtCurrent() { e.evaluate(true, dParams + kHmcTau); }
+
+void Hmc::buildGraph() {
+    if (graphExec) return;
+    const bool timing = e.timing;
+    e.timing = false;
+    check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    leapfrogStep();
+    check(hipStreamEndCapture(e.stream, &graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    e.timing = timing;
+}
+
+// energies at the current locations: hScalars[0] = K, [1] = x^T T x sum,
+// [2] = sumOfIncrements (0.5 * sum of tile partials)
+void Hmc::readEnergies() {
+    if (e.fp32)
+        hipLaunchKernelGGL((energy_kernel<float>), dim3(1), dim3(kEnergyThreads), 0, e.stream,
+                           (const float*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, dEnergy);
+    else
+        hipLaunchKernelGGL((energy_kernel<double>), dim3(1), dim3(kEnergyThreads), 0, e.stream,
+                           (const double*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, dEnergy);
+    check(hipGetLastError(), "energy kernel");
+    check(hipMemcpyAsync(hScalars, dEnergy, 2 * sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
+    check(hipMemcpyAsync(hScalars + 2, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
+    check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+}
+
+double Hmc::logLikelihood(double rawSum, double t) const {
+    // rawSum = 0.5 * sum of tile partials; truncated: sumOfIncrements itself,
+    // otherwise scaled by 0.5 tau (src/NewMultiDimensionalScaling.hpp:198-202)
+    const double s = e.truncated ? rawSum : 0.5 * t * rawSum;
+    const double pairs = (double)n * (n - 1) / 2.0;
+    return 0.5 * (std::log(t) - std::log(2.0 * M_PI)) * pairs - s;
+}
+
+double Hmc::logPrior(double quad) const {
+    return -0.5 * quad - 0.5 * n * p * std::log(2.0 * M_PI) + 0.5 * n * log_det_spd(trait, p);
+}
+
+void Hmc::copyLocationsToHost(std::vector<double>& out) {
+    const size_t bytes = (size_t)n * e.kdim * e.realSize;
+    check(hipMemcpyAsync(hX, e.dX[e.cur], bytes, hipMemcpyDeviceToHost, e.stream), "hipMemcpy(locations)");
+    check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+    out.resize((size_t)n * p);
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < p; ++d)
+            out[(size_t)i * p + d] = e.fp32 ? (double)((const float*)hX)[(size_t)i * e.kdim + d]
+                                            : ((const double*)hX)[(size_t)i * e.kdim + d];
+}
+
+void Hmc::record(double* row, const std::vector<double>& x, double logPost) const {
+    row[0] = logPost;
+    row[1] = tau;
+    double tr = 0.0;
+    for (int d = 0; d < p; ++d) tr += trait[d * p + d];
+    row[2] = tr;
+    const int pairs[5][2] = {{0, 1}, {0, n / 2}, {n / 3, 2 * n / 3}, {1, n - 1}, {n / 4, 3 * n / 4}};
+    for (int k = 0; k < 5; ++k) {
+        double s = 0.0;
+        for (int d = 0; d < p; ++d) {
+            const double v = x[(size_t)pairs[k][0] * p + d] - x[(size_t)pairs[k][1] * p + d];
+            s += v * v;
+        }
+        row[3 + k] = std::sqrt(s);
+    }
+    const int rows[3] = {0, n / 2, n - 1};
+    for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
+        for (int d = 0; d < p; ++d) s += x[(size_t)rows[k] * p + d] * x[(size_t)rows[k] * p + d];
+        row[8 + k] = std::sqrt(s);
+    }
+}
+
+void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats* stats) {
+    using clock = std::chrono::steady_clock;
+    const auto tStart = clock::now();
+    mdsgen::Stream rng(opt.seed, 77);
+    e.updateLocations(-1, x0, (size_t)n * p);
+    uploadParams();
+    gradientAtCurrent();
+    check(hipMemsetAsync(dMom, 0, (size_t)n * p * sizeof(double), e.stream), "hipMemset(momentum)");
+    readEnergies();
+    double logLik = logLikelihood(hScalars[2], tau);
+    double quad = hScalars[1];
+    double U = -(logLik + logPrior(quad));
+    if (opt.useGraph) buildGraph();
+
+    const size_t xBytes = (size_t)e.npad * e.kdim * e.realSize;
+    const size_t gBytes = (size_t)n * p * sizeof(double);
+    long long accepted = 0, proposedTau = 0, acceptedTau = 0, evaluations = 1;
+    std::vector<double> xHost;
+    auto tSampling = clock::now();
+    const int threads = 256;
+    for (int it = 0; it < opt.iterations; ++it) {
+        if (it == opt.burnIn) tSampling = clock::now();
+        const int count = n * p;
+        hipLaunchKernelGGL(momentum_kernel, dim3((count / 2 + threads) / threads), dim3(threads), 0, e.stream, dMom,
+                           count, (uint64_t)opt.seed, (uint64_t)it);
+        check(hipGetLastError(), "momentum kernel");
+        readEnergies();
+        const double K0 = hScalars[0];
+        check(hipMemcpyAsync(dXSave, e.dX[e.cur], xBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
+        check(hipMemcpyAsync(dGradSave, e.dGrad, gBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
+        for (int s = 0; s < opt.leapfrogSteps; ++s) {
+            if (graphExec)
+                check(hipGraphLaunch(graphExec, e.stream), "hipGraphLaunch");
+            else
+                leapfrogStep();
+        }
+        evaluations += opt.leapfrogSteps;
+        readEnergies();
+        const double K1 = hScalars[0];
+        const double logLik1 = logLikelihood(hScalars[2], tau);
+        const double quad1 = hScalars[1];
+        const double U1 = -(logLik1 + logPrior(quad1));
+        const double ratio = (U + K0) - (U1 + K1);  // R/Untitled.R:630-637
+        const double u = rng.uniform();
+        if (std::isfinite(ratio) && ratio > std::min(0.0, std::log(u))) {
+            U = U1;
+            logLik = logLik1;
+            quad = quad1;
+            ++accepted;
+        } else {
+            check(hipMemcpyAsync(e.dX[e.cur], dXSave, xBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
+            check(hipMemcpyAsync(e.dGrad, dGradSave, gBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
+        }
+
+        if (opt.learnPrecision) {
+            if (e.truncated) {
+                // random-walk Metropolis at the current locations
+                const double lo = tau - opt.precisionProposalWidth, hi = tau + opt.precisionProposalWidth;
+                const double tauStar = std::fabs(lo + (hi - lo) * rng.uniform());
+                ++proposedTau;
+                if (tauStar > 0.0) {
+                    const double saved = tau;
+                    tau = tauStar;
+                    uploadParams();
+                    e.evaluate(false, dParams + kHmcTau);
+                    ++evaluations;
+                    check(hipMemcpyAsync(hScalars + 2, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream),
+                          "hipMemcpy");
+                    check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+                    const double logLikStar = logLikelihood(hScalars[2], tauStar);
+                    const double r = logLikStar - logLik;  // the prior does not involve tau
+                    if (std::isfinite(r) && r > std::min(0.0, std::log(rng.uniform()))) {
+                        logLik = logLikStar;
+                        U = -(logLik + logPrior(quad));
+                        ++acceptedTau;
+                        gradientAtCurrent();  // gradient at the new tau
+                        ++evaluations;
+                    } else {
+                        // the likelihood-only pass left the gradient at the old
+                        // tau in place
+                        tau = saved;
+                        uploadParams();
+                    }
+                }
+            } else {
+                // Gibbs: tau | X ~ Gamma(1 + N(N-1)/4, C / 2) with C the unscaled
+                // sum of squared residuals (this engine's raw sum at the current X)
+                gradientAtCurrent();
+                ++evaluations;
+                check(hipMemcpyAsync(hScalars + 2, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream),
+                      "hipMemcpy");
+                check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+                const double C = hScalars[2];
+                const double pairs = (double)n * (n - 1) / 2.0;
+                tau = gamma_draw(rng, 1.0 + 0.5 * pairs) / (0.5 * C);
+                ++proposedTau;
+                ++acceptedTau;
+                uploadParams();
+                gradientAtCurrent();
+                ++evaluations;
+                logLik = logLikelihood(C, tau);
+                U = -(logLik + logPrior(quad));
+            }
+        }
+
+        const bool traitStep = opt.learnTraitPrecision && it < opt.iterations - 1 &&
+                               ((it + 1) % opt.traitInvWeight == 0);
+        if (traitStep || it >= opt.burnIn) copyLocationsToHost(xHost);
+        if (traitStep) {
+            std::vector<double> m((size_t)p * p, 0.0);  // T0 + X^T X, T0 = I_P
+            for (int d = 0; d < p; ++d) m[d * p + d] = 1.0;
+            for (int i = 0; i < n; ++i)
+                for (int a = 0; a < p; ++a)
+                    for (int b = 0; b < p; ++b) m[a * p + b] += xHost[(size_t)i * p + a] * xHost[(size_t)i * p + b];
+            const double df = (opt.priorDf > 0 ? opt.priorDf : p) + n;
+            trait = wishart_draw(rng, df, spd_inverse(m, p), p);
+            uploadParams();
+            quad = 0.0;
+            for (int i = 0; i < n; ++i)
+                for (int a = 0; a < p; ++a)
+                    for (int b = 0; b < p; ++b)
+                        quad += xHost[(size_t)i * p + a] * trait[a * p + b] * xHost[(size_t)i * p + b];
+            U = -(logLik + logPrior(quad));
+        }
+        if (it >= opt.burnIn && trace) record(trace + (size_t)(it - opt.burnIn) * kHmcTraceColumns, xHost, -U);
+    }
+    check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+    const auto tEnd = clock::now();
+    if (finalX) {
+        copyLocationsToHost(xHost);
+        std::memcpy(finalX, xHost.data(), xHost.size() * sizeof(double));
+    }
+    // leave the engine consistent: new precision, caches recomputed on demand
+    e.precision = tau;
+    e.makeDirty();
+    if (stats) {
+        stats->acceptRate = (double)accepted / opt.iterations;
+        stats->precisionAcceptRate = proposedTau ? (double)acceptedTau / proposedTau : 0.0;
+        stats->samplingSeconds = std::chrono::duration<double>(tEnd - tSampling).count();
+        stats->totalSeconds = std::chrono::duration<double>(tEnd - tStart).count();
+        stats->evaluations = (double)evaluations;
+        stats->finalPrecision = tau;
+        stats->stepSize = eps;
+        stats->finalPotential = U;
+    }
+}
+
+}  // namespace mdship
+
+extern "C" {
+
+int mds_hip_hmc_default_options(mds_hip_hmc_options* o) {
+    if (!o) return MDS_HIP_ERR_ARGUMENT;
+    o->iterations = 2000;
+    o->burnIn = 500;
+    o->leapfrogSteps = 20;
+    o->trajectory = 0.1;
+    o->learnPrecision = 1;
+    o->learnTraitPrecision = 1;
+    o->traitInvWeight = 1;
+    o->precisionProposalWidth = 0.01;
+    o->priorDf = 0.0;
+    o->seed = 666;
+    o->useGraph = 1;
+    return MDS_HIP_OK;
+}
+
+int mds_hip_hmc_run(mds_hip_handle h, const mds_hip_hmc_options* options, const double* initialLocations,
+                    size_t length, double* trace, size_t traceLength, double* finalLocations,
+                    mds_hip_hmc_stats* stats) {
+    if (!h || !h->e) return MDS_HIP_ERR_ARGUMENT;
+    mdship::Engine& e = *h->e;
+    try {
+        (void)hipSetDevice(e.device);
+        if (!options || !initialLocations || length != (size_t)e.N * e.P)
+            throw mdship::Error(MDS_HIP_ERR_ARGUMENT, "hmc: options and N*P initial locations required");
+        if (trace && traceLength != (size_t)(options->iterations - options->burnIn) * mdship::kHmcTraceColumns)
+            throw mdship::Error(MDS_HIP_ERR_ARGUMENT, "hmc: trace length must be (iterations - burnIn) * columns");
+        mdship::Hmc hmc(e, *options);
+        hmc.run(initialLocations, trace, finalLocations, stats);
+        return MDS_HIP_OK;
+    } catch (const mdship::Error& err) {
+        e.lastError = err.what();
+        return err.code;
+    } catch (const std::exception& err) {
+        e.lastError = err.what();
+        return MDS_HIP_ERR_DEVICE;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+// Device-resident HMC sampler over an Engine (see mds_hmc.hip).
+#pragma once
+
+#include <vector>
+
+#include "mds_engine.hpp"
+#include "mds_hip.h"
+
+namespace mdship {
+
+// Device parameter block: precision, step size, trait precision (P x P).
+constexpr int kHmcTau = 0;
+constexpr int kHmcEps = 1;
+constexpr int kHmcTrait = 2;
+constexpr int kHmcParamCount = kHmcTrait + kMaxDim * kMaxDim;
+constexpr int kHmcTraceColumns = MDS_HIP_HMC_TRACE_COLUMNS;
+
+struct Hmc {
+    Engine& e;
+    mds_hip_hmc_options opt;
+    int n = 0, p = 0;
+    double tau = 0.0, eps = 0.0;
+    std::vector<double> trait;
+    double* dMom = nullptr;
+    double* dGradSave = nullptr;
+    void* dXSave = nullptr;
+    double* dParams = nullptr;
+    double* dEnergy = nullptr;
+    double* hParams = nullptr;
+    double* hScalars = nullptr;
+    void* hX = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+
+    Hmc(Engine& engine, const mds_hip_hmc_options& options);
+    ~Hmc();
+    void uploadParams();
+    void leapfrogStep();
+    void gradientAtCurrent();
+    void buildGraph();
+    void readEnergies();
+    double logLikelihood(double rawSum, double t) const;
+    double logPrior(double quad) const;
+    void copyLocationsToHost(std::vector<double>& out);
+    void record(double* row, const std::vector<double>& x, double logPost) const;
+    void run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats* stats);
+};
+
+}  // namespace mdship

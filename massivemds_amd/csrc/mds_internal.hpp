@@ -39,6 +39,7 @@ struct TileLaunch {
     bool gradient;
     double* tileSums;        // [numTiles]
     double* part;            // [nb][nb][kdim][64] gradient partials
+    const double* devTau = nullptr;  // if set, the precision is read from here
 };
 
 struct ReduceLaunch {

@@ -27,9 +27,16 @@ __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
     const Real* __restrict__ delta, const Real* __restrict__ x,
     const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
     Real tau, Real halfTau, Real sqrtTau, Real cg, double* __restrict__ tileSums,
-    double* __restrict__ part) {
+    double* __restrict__ part, const double* __restrict__ devTau) {
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
     if (t >= numTiles) return;
+    if (devTau) {  // same arithmetic as the host-side derivation in go()
+        const double v = *devTau;
+        tau = (Real)v;
+        halfTau = (Real)(0.5 * v);
+        sqrtTau = (Real)sqrt(v);
+        cg = (Real)(kInvSqrt2Pi / sqrt(v));
+    }
     const uint32_t code = tiles[t];
     tile_body<Real, P, TRUNC, SYM, GRAD, kGroup>(delta + (int64_t)t * kTileElems, planeStride, x,
                                                  (int)(code >> 16), (int)(code & 0xffffu), nb, tau, halfTau,
@@ -97,7 +104,7 @@ static hipError_t go(const TileLaunch& a, hipStream_t s) {
     const Real cg = (Real)(kInvSqrt2Pi / sqrt(a.tau));
     hipLaunchKernelGGL((tile_kernel<Real, P, TRUNC, SYM, GRAD>), grid, dim3(kBlockThreads), 0, s,
                        (const Real*)a.delta, (const Real*)a.x, a.tiles, a.numTiles, a.nb,
-                       a.planeStride, tau, halfTau, sqrtTau, cg, a.tileSums, a.part);
+                       a.planeStride, tau, halfTau, sqrtTau, cg, a.tileSums, a.part, a.devTau);
     return hipGetLastError();
 }
 
