@@ -35,6 +35,23 @@ HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
 
 
+def _measured_traffic():
+    """HBM bytes per FP64 tile-kernel launch at this workload, from the
+    rocprofv3 FETCH_SIZE + WRITE_SIZE runs summarised in
+    profiles/tile_kernel_n10000_p2.md (counters scaled by a plain copy's
+    reading); None when the summary is absent."""
+    from pathlib import Path
+
+    path = Path(__file__).resolve().parent / "profiles" / "tile_kernel_n10000_p2.md"
+    try:
+        for line in path.read_text().splitlines():
+            if line.startswith("traffic_bytes_fp64_tile:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def _workload():
     from massivemds_amd import datagen
 
@@ -276,7 +293,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": _measured_traffic(),
         },
         "fp32": {
             "value": 1e3 / r32["wall_ms_per_step"],
