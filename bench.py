@@ -216,6 +216,40 @@ def _bench_hmc(iterations=2000, burn_in=500, n=5000, p=2):
     print(json.dumps(out))
 
 
+def _bench_surrogate(iterations=500, n=20000, p=5):
+    """Surrogate-trajectory HMC against plain HMC: N = 20000, P = 5, FP64,
+    euclid10 with 2% missing, truncation, precision fixed at its plug-in value
+    at the posterior mode, trait precision I.  Warm-up: 100 plain iterations
+    of 20 leapfrog steps (2000 exact gradient evaluations, all stored as
+    training snapshots) with dual-averaged step size; training: 4 epochs over
+    the snapshots on the GPU; then `iterations` of each sampler from the same
+    state and step size.  Reports min-ESS/s of both, their ratio and the
+    largest |difference| of posterior means in Monte Carlo standard errors."""
+    from massivemds_amd import datagen
+    from massivemds_amd.surrogate import compare_samplers
+
+    delta = datagen.euclid10(n, missing=0.02)
+    r = compare_samplers(delta, p, iterations, warmup_evals=2000, leapfrog=20, truncation=True, epochs=4)
+    del delta
+    runs = {}
+    for name in ("plain", "surrogate"):
+        v = r[name]
+        runs[name] = {k: (round(float(v[k]), 4) if isinstance(v[k], float) else v[k])
+                      for k in ("accept_rate", "seconds", "ms_per_iteration", "min_ess", "min_ess_per_s",
+                                "surrogate_iterations", "exact_iterations", "surrogate_accept_rate")}
+    out = {
+        "metric": "min-ESS/s surrogate vs plain HMC", "value": round(r["ess_per_s_ratio"], 4), "unit": "ratio",
+        "higher_is_better": True, "dtype": "f64 likelihood, bf16 network",
+        "config": {"workload": f"surrogate_hmc_n{n}_p{p}_fp64_trunc_missing2", "N": n, "P": p,
+                   "iterations": iterations, "leapfrog": 20, "warmup_evaluations": 2000, "epochs": 4},
+        "precision": r["precision"], "step_size": r["step_size"], "warmup_accept": r["warmup_accept"],
+        "snapshots": r["snapshots"], "train_loss": r["train_loss"], "map_s": r["map_s"],
+        "warmup_s": r["warmup_s"], "train_s": r["train_s"], "max_mean_z": r["max_mean_z"],
+        "plain": runs["plain"], "surrogate": runs["surrogate"],
+    }
+    print(json.dumps(out))
+
+
 def _calibrate_copy(mib):
     """A plain elementwise copy kernel of known size (torch, as plumbing)."""
     import torch
@@ -237,9 +271,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--workload", default="n10000", choices=["n10000", "batch64", "hmc5000"],
+    ap.add_argument("--workload", default="n10000", choices=["n10000", "batch64", "hmc5000", "surrogate20000"],
                     help="n10000: the metric's workload (default); batch64: the uneven batch; "
-                         "hmc5000: plain HMC")
+                         "hmc5000: plain HMC; surrogate20000: surrogate against plain HMC")
+    ap.add_argument("--iterations", type=int, default=500, help="sampling iterations (surrogate20000)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch64)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to scale "
@@ -253,6 +288,9 @@ def main():
         return
     if args.workload == "hmc5000":
         _bench_hmc()
+        return
+    if args.workload == "surrogate20000":
+        _bench_surrogate(args.iterations)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a "

@@ -179,6 +179,45 @@ int mds_hip_batch_get_timing(mds_hip_batch_handle h, double* ms, long long* coun
  */
 #define MDS_HIP_HMC_TRACE_COLUMNS 11
 
+/*
+ * Surrogate of the gradient rows (no counterpart in the reference; for
+ * samplers that can afford approximate leapfrog gradients because the exact
+ * likelihood still decides acceptance): a per-location MLP, input x_i (P <= 16)
+ * plus a learned 16-value embedding of location i, two hidden layers of 256
+ * with tanh, output the P gradient components; BF16 weights on the MFMA
+ * matrix cores with FP32 accumulation, trained on the GPU with Adam on
+ * (x, exact gradient) snapshots.  Snapshots come from HMC runs with
+ * collectSnapshots set, or from mds_hip_surrogate_add_snapshot.
+ */
+typedef struct mds_hip_surrogate* mds_hip_surrogate_handle;
+
+int mds_hip_surrogate_create(int locationCount, int embeddingDimension, int snapshotCapacity,
+                             unsigned long long seed, int device, mds_hip_surrogate_handle* out);
+int mds_hip_surrogate_destroy(mds_hip_surrogate_handle h);
+const char* mds_hip_surrogate_last_error(mds_hip_surrogate_handle h);
+int mds_hip_surrogate_snapshot_count(mds_hip_surrogate_handle h, int* count);
+int mds_hip_surrogate_add_snapshot(mds_hip_surrogate_handle h, const double* x, const double* g, size_t length);
+/* Trains on all stored snapshots; finalLoss: mean squared error on the
+ * normalised targets after training. */
+int mds_hip_surrogate_train(mds_hip_surrogate_handle h, int epochs, double learningRate, double* finalLoss);
+/* Predicted log-likelihood gradient rows (N x P) at locations x (N x P). */
+int mds_hip_surrogate_predict(mds_hip_surrogate_handle h, const double* x, size_t length, double* grad);
+/* FP32 parameters: W1 (32 x 256), b1 (256), W2 (256 x 256), b2 (256), W3 (256 x 16),
+ * b3 (16), embeddings (N x 16), all row-major, concatenated. */
+int mds_hip_surrogate_parameter_count(mds_hip_surrogate_handle h, long long* count);
+int mds_hip_surrogate_get_parameters(mds_hip_surrogate_handle h, float* out, size_t count);
+int mds_hip_surrogate_set_parameters(mds_hip_surrogate_handle h, const float* in, size_t count);
+/* Normalisation (train sets it from the snapshots): network input
+ * (x_i - xCentre_i) / sigmaX and target (g_i - gCentre_i) / sigmaG, with
+ * per-location centres (N x P each) and per-dimension scales (P each). */
+int mds_hip_surrogate_set_normalisation(mds_hip_surrogate_handle h, const float* xCentre, const float* gCentre,
+                                        size_t centreLength, const float* sigmaX, const float* sigmaG,
+                                        size_t scaleLength);
+/* One forward + backward pass without an update: the gradient of the loss
+ * with respect to every parameter (same order as get_parameters) and the loss. */
+int mds_hip_surrogate_loss_gradient(mds_hip_surrogate_handle h, const double* x, const double* g, size_t length,
+                                    float* paramGrad, size_t count, double* loss);
+
 typedef struct {
     int iterations;
     int burnIn;
@@ -191,6 +230,18 @@ typedef struct {
     double priorDf;              /* <= 0: P */
     unsigned long long seed;
     int useGraph;
+    int adaptStepSize;           /* dual averaging (Hoffman & Gelman 2014) during burn-in */
+    double targetAccept;         /* its target acceptance rate */
+    double stepSize;             /* > 0: leapfrog step, overriding trajectory / leapfrogSteps */
+    mds_hip_surrogate_handle surrogate;
+    int useSurrogate;            /* leapfrog on the surrogate's gradients; exact likelihood in the test */
+    int collectSnapshots;        /* store (x, exact gradient) after every exact gradient pass
+                                  * at the start and along accepted trajectories (a rejected
+                                  * trajectory's snapshots are dropped) */
+    double fallbackAcceptRate;   /* surrogate: below this acceptance over a window of */
+    int fallbackWindow;          /* iterations, exact gradients take over for fallbackWindow
+                                  * iterations, doubled after each consecutive failed window
+                                  * (at most 1024x); 0 disables the fallback */
 } mds_hip_hmc_options;
 
 typedef struct {
@@ -202,6 +253,10 @@ typedef struct {
     double finalPrecision;
     double stepSize;
     double finalPotential;
+    double surrogateIterations;  /* iterations whose trajectory used the surrogate */
+    double exactIterations;
+    double snapshots;            /* snapshots stored in this run */
+    double surrogateAcceptRate;  /* acceptance over the surrogate iterations alone */
 } mds_hip_hmc_stats;
 
 int mds_hip_hmc_default_options(mds_hip_hmc_options* options);

@@ -37,6 +37,7 @@
 #include "mds_device.hpp"
 #include "mds_engine.hpp"
 #include "mds_hmc.hpp"
+#include "mds_mlp.hpp"
 
 namespace mdship {
 
@@ -251,13 +252,23 @@ Hmc::Hmc(Engine& engine, const mds_hip_hmc_options& options) : e(engine), opt(op
     tau = e.precision;
     trait.assign((size_t)p * p, 0.0);
     for (int d = 0; d < p; ++d) trait[d * p + d] = 1.0;  // solve(traitVcv) with traitVcv = I_P
-    eps = opt.trajectory / opt.leapfrogSteps;
+    eps = opt.stepSize > 0.0 ? opt.stepSize : opt.trajectory / opt.leapfrogSteps;
+    if (opt.useSurrogate || opt.collectSnapshots) {
+        if (!opt.surrogate || !opt.surrogate->s)
+            throw Error(MDS_HIP_ERR_ARGUMENT, "surrogate handle required for useSurrogate / collectSnapshots");
+        sur = opt.surrogate->s;
+        if (sur->mlp.n != n || sur->mlp.p != p)
+            throw Error(MDS_HIP_ERR_ARGUMENT, "surrogate built for another N or P");
+        if (sur->mlp.device != e.device) throw Error(MDS_HIP_ERR_ARGUMENT, "surrogate on another device");
+    }
 }
 
 Hmc::~Hmc() {
     (void)hipStreamSynchronize(e.stream);
-    if (graphExec) (void)hipGraphExecDestroy(graphExec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (int m = 0; m < 3; ++m) {
+        if (graphExec[m]) (void)hipGraphExecDestroy(graphExec[m]);
+        if (graph[m]) (void)hipGraphDestroy(graph[m]);
+    }
     (void)hipFree(dMom);
     (void)hipFree(dGradSave);
     (void)hipFree(dXSave);
@@ -276,7 +287,7 @@ void Hmc::uploadParams() {
           "hipMemcpy(parameters)");
 }
 
-void Hmc::leapfrogStep() {
+void Hmc::leapfrogStep(int mode) {
     const int threads = 256, blocks = (n + threads - 1) / threads;
     if (e.fp32) {
         hipLaunchKernelGGL((kick_kernel<float, true>), dim3(blocks), dim3(threads), 0, e.stream, (float*)e.dX[e.cur],
@@ -286,7 +297,7 @@ void Hmc::leapfrogStep() {
                            (double*)e.dX[e.cur], dMom, e.dGrad, dParams, n, p, e.kdim);
     }
     check(hipGetLastError(), "kick/drift kernel");
-    gradientAtCurrent();
+    gradient(mode);
     if (e.fp32) {
         hipLaunchKernelGGL((kick_kernel<float, false>), dim3(blocks), dim3(threads), 0, e.stream,
                            (float*)e.dX[e.cur], dMom, e.dGrad, dParams, n, p, e.kdim);
@@ -299,14 +310,30 @@ void Hmc::leapfrogStep() {
 
 void Hmc::gradientAtCurrent() { e.evaluate(true, dParams + kHmcTau); }
 
-void Hmc::buildGraph() {
-    if (graphExec) return;
+// Gradient of the log-likelihood at the current locations into e.dGrad:
+// exact (fused kernels; with kExactCollect the (x, gradient) pair is also
+// stored for the surrogate's training), or the surrogate's prediction.
+void Hmc::gradient(int mode) {
+    if (mode == kSurrogate) {
+        if (e.fp32)
+            sur->mlp.predictInto<float>((const float*)e.dX[e.cur], e.kdim, e.dGrad, e.stream);
+        else
+            sur->mlp.predictInto<double>((const double*)e.dX[e.cur], e.kdim, e.dGrad, e.stream);
+        return;
+    }
+    gradientAtCurrent();
+    if (mode == kExactCollect)
+        launch_snapshot(e, e.dGrad, sur->store.sx, sur->store.sg, sur->store.dSlot, sur->store.capacity, e.stream);
+}
+
+void Hmc::buildGraph(int mode) {
+    if (graphExec[mode]) return;
     const bool timing = e.timing;
     e.timing = false;
     check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    leapfrogStep();
-    check(hipStreamEndCapture(e.stream, &graph), "hipStreamEndCapture");
-    check(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    leapfrogStep(mode);
+    check(hipStreamEndCapture(e.stream, &graph[mode]), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&graphExec[mode], graph[mode], nullptr, nullptr, 0), "hipGraphInstantiate");
     e.timing = timing;
 }
 
@@ -377,22 +404,43 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
     mdsgen::Stream rng(opt.seed, 77);
     e.updateLocations(-1, x0, (size_t)n * p);
     uploadParams();
-    gradientAtCurrent();
+    const int startSnapshots = sur ? sur->store.count() : 0;
+    const int exactMode = opt.collectSnapshots ? kExactCollect : kExact;
+    int gradMode = exactMode;  // source of the gradient now in e.dGrad
+    gradient(exactMode);
     check(hipMemsetAsync(dMom, 0, (size_t)n * p * sizeof(double), e.stream), "hipMemset(momentum)");
     readEnergies();
     double logLik = logLikelihood(hScalars[2], tau);
     double quad = hScalars[1];
     double U = -(logLik + logPrior(quad));
-    if (opt.useGraph) buildGraph();
+
+    // dual averaging of the step size (Hoffman & Gelman 2014, sec. 3.2)
+    const double daMu = std::log(10.0 * eps), daGamma = 0.05, daT0 = 10.0, daKappa = 0.75;
+    double daH = 0.0, daLogEpsBar = 0.0;
 
     const size_t xBytes = (size_t)e.npad * e.kdim * e.realSize;
     const size_t gBytes = (size_t)n * p * sizeof(double);
     long long accepted = 0, proposedTau = 0, acceptedTau = 0, evaluations = 1;
+    long long surrogateIts = 0, exactIts = 0, surrogateAccepted = 0;
+    int windowIts = 0, windowAcc = 0, fallbackLeft = 0, failedWindows = 0;
     std::vector<double> xHost;
     auto tSampling = clock::now();
     const int threads = 256;
     for (int it = 0; it < opt.iterations; ++it) {
-        if (it == opt.burnIn) tSampling = clock::now();
+        if (it == opt.burnIn) {
+            tSampling = clock::now();
+            if (opt.adaptStepSize && opt.burnIn > 0) {
+                eps = std::exp(daLogEpsBar);
+                uploadParams();
+            }
+        }
+        const int mode = (opt.useSurrogate && fallbackLeft == 0) ? kSurrogate : exactMode;
+        if (mode != gradMode && !(mode != kSurrogate && gradMode != kSurrogate)) {
+            gradient(mode);  // the trajectory starts from this source's gradient
+            if (mode != kSurrogate) ++evaluations;
+        }
+        gradMode = mode;
+        if (opt.useGraph) buildGraph(mode);
         const int count = n * p;
         hipLaunchKernelGGL(momentum_kernel, dim3((count / 2 + threads) / threads), dim3(threads), 0, e.stream, dMom,
                            count, (uint64_t)opt.seed, (uint64_t)it);
@@ -401,13 +449,24 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
         const double K0 = hScalars[0];
         check(hipMemcpyAsync(dXSave, e.dX[e.cur], xBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
         check(hipMemcpyAsync(dGradSave, e.dGrad, gBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
+        if (mode == kExactCollect)  // snapshots of a rejected trajectory are dropped
+            check(hipMemcpyAsync(sur->store.dSlot + 1, sur->store.dSlot, sizeof(double), hipMemcpyDeviceToDevice,
+                                 e.stream), "hipMemcpy(slot)");
         for (int s = 0; s < opt.leapfrogSteps; ++s) {
-            if (graphExec)
-                check(hipGraphLaunch(graphExec, e.stream), "hipGraphLaunch");
+            if (graphExec[mode])
+                check(hipGraphLaunch(graphExec[mode], e.stream), "hipGraphLaunch");
             else
-                leapfrogStep();
+                leapfrogStep(mode);
         }
-        evaluations += opt.leapfrogSteps;
+        if (mode == kSurrogate) {
+            // the exact likelihood (FP64-reduced) judges the proposal
+            e.evaluate(false, dParams + kHmcTau);
+            ++evaluations;
+            ++surrogateIts;
+        } else {
+            evaluations += opt.leapfrogSteps;
+            ++exactIts;
+        }
         readEnergies();
         const double K1 = hScalars[0];
         const double logLik1 = logLikelihood(hScalars[2], tau);
@@ -415,7 +474,9 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
         const double U1 = -(logLik1 + logPrior(quad1));
         const double ratio = (U + K0) - (U1 + K1);  // R/Untitled.R:630-637
         const double u = rng.uniform();
-        if (std::isfinite(ratio) && ratio > std::min(0.0, std::log(u))) {
+        const bool acc = std::isfinite(ratio) && ratio > std::min(0.0, std::log(u));
+        if (acc && mode == kSurrogate) ++surrogateAccepted;
+        if (acc) {
             U = U1;
             logLik = logLik1;
             quad = quad1;
@@ -423,9 +484,46 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
         } else {
             check(hipMemcpyAsync(e.dX[e.cur], dXSave, xBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
             check(hipMemcpyAsync(e.dGrad, dGradSave, gBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
+            if (mode == kExactCollect)
+                check(hipMemcpyAsync(sur->store.dSlot, sur->store.dSlot + 1, sizeof(double), hipMemcpyDeviceToDevice,
+                                     e.stream), "hipMemcpy(slot)");
+        }
+        if (opt.adaptStepSize && it < opt.burnIn) {
+            const double alpha = std::isfinite(ratio) ? std::min(1.0, std::exp(std::min(ratio, 0.0))) : 0.0;
+            const double m = it + 1.0;
+            daH = (1.0 - 1.0 / (m + daT0)) * daH + (opt.targetAccept - alpha) / (m + daT0);
+            const double logEps = daMu - std::sqrt(m) / daGamma * daH;
+            const double eta = std::pow(m, -daKappa);
+            daLogEpsBar = eta * logEps + (1.0 - eta) * daLogEpsBar;
+            eps = std::exp(logEps);
+            uploadParams();
+        }
+        if (opt.useSurrogate && opt.fallbackWindow > 0) {
+            if (fallbackLeft > 0) {
+                --fallbackLeft;
+            } else {
+                ++windowIts;
+                windowAcc += acc ? 1 : 0;
+                if (windowIts == opt.fallbackWindow) {
+                    // a failed window hands over to exact gradients for a
+                    // window that doubles with each consecutive failure
+                    if ((double)windowAcc / windowIts < opt.fallbackAcceptRate) {
+                        fallbackLeft = opt.fallbackWindow << std::min(failedWindows, 10);
+                        ++failedWindows;
+                    } else {
+                        failedWindows = 0;
+                    }
+                    windowIts = windowAcc = 0;
+                }
+            }
         }
 
         if (opt.learnPrecision) {
+            if (gradMode == kSurrogate) {
+                gradient(exactMode);  // precision moves use the exact likelihood and gradient
+                gradMode = exactMode;
+                ++evaluations;
+            }
             if (e.truncated) {
                 // random-walk Metropolis at the current locations
                 const double lo = tau - opt.precisionProposalWidth, hi = tau + opt.precisionProposalWidth;
@@ -515,6 +613,10 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
         stats->finalPrecision = tau;
         stats->stepSize = eps;
         stats->finalPotential = U;
+        stats->surrogateIterations = (double)surrogateIts;
+        stats->exactIterations = (double)exactIts;
+        stats->snapshots = sur ? (double)(sur->store.count() - startSnapshots) : 0.0;
+        stats->surrogateAcceptRate = surrogateIts ? (double)surrogateAccepted / surrogateIts : 0.0;
     }
 }
 
@@ -535,6 +637,14 @@ int mds_hip_hmc_default_options(mds_hip_hmc_options* o) {
     o->priorDf = 0.0;
     o->seed = 666;
     o->useGraph = 1;
+    o->adaptStepSize = 0;
+    o->targetAccept = 0.8;
+    o->stepSize = 0.0;
+    o->surrogate = nullptr;
+    o->useSurrogate = 0;
+    o->collectSnapshots = 0;
+    o->fallbackAcceptRate = 0.2;
+    o->fallbackWindow = 20;
     return MDS_HIP_OK;
 }
 

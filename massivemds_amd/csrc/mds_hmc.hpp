@@ -5,6 +5,7 @@
 
 #include "mds_engine.hpp"
 #include "mds_hip.h"
+#include "mds_mlp.hpp"
 
 namespace mdship {
 
@@ -29,15 +30,19 @@ struct Hmc {
     double* hParams = nullptr;
     double* hScalars = nullptr;
     void* hX = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graphExec = nullptr;
+    // gradient sources of a leapfrog step
+    enum Mode { kExact = 0, kExactCollect = 1, kSurrogate = 2 };
+    Surrogate* sur = nullptr;
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t graphExec[3] = {nullptr, nullptr, nullptr};
 
     Hmc(Engine& engine, const mds_hip_hmc_options& options);
     ~Hmc();
     void uploadParams();
-    void leapfrogStep();
+    void leapfrogStep(int mode);
     void gradientAtCurrent();
-    void buildGraph();
+    void gradient(int mode);
+    void buildGraph(int mode);
     void readEnergies();
     double logLikelihood(double rawSum, double t) const;
     double logPrior(double quad) const;

@@ -36,6 +36,14 @@ class HmcOptions(ctypes.Structure):
         ("priorDf", ctypes.c_double),
         ("seed", ctypes.c_ulonglong),
         ("useGraph", ctypes.c_int),
+        ("adaptStepSize", ctypes.c_int),
+        ("targetAccept", ctypes.c_double),
+        ("stepSize", ctypes.c_double),
+        ("surrogate", ctypes.c_void_p),
+        ("useSurrogate", ctypes.c_int),
+        ("collectSnapshots", ctypes.c_int),
+        ("fallbackAcceptRate", ctypes.c_double),
+        ("fallbackWindow", ctypes.c_int),
     ]
 
 
@@ -49,6 +57,10 @@ class HmcStats(ctypes.Structure):
         ("finalPrecision", ctypes.c_double),
         ("stepSize", ctypes.c_double),
         ("finalPotential", ctypes.c_double),
+        ("surrogateIterations", ctypes.c_double),
+        ("exactIterations", ctypes.c_double),
+        ("snapshots", ctypes.c_double),
+        ("surrogateAcceptRate", ctypes.c_double),
     ]
 
 
@@ -59,6 +71,14 @@ def _bind(lib):
                                     _c_double_p, ctypes.c_size_t, _c_double_p, ctypes.POINTER(HmcStats)]
     lib.mds_hip_hmc_run.restype = ctypes.c_int
     return lib
+
+
+def default_options(**overrides) -> HmcOptions:
+    opt = HmcOptions()
+    _bind(load_library()).mds_hip_hmc_default_options(ctypes.byref(opt))
+    for k, v in overrides.items():
+        setattr(opt, k, v)
+    return opt
 
 
 def run_hmc(engine: MultiDimensionalScaling, initial, options: HmcOptions):
