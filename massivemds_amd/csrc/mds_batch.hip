@@ -64,44 +64,39 @@ struct ReduceItem {
 constexpr int kBatchMaxThreads = 512;
 
 template <typename Real, int P>
-__device__ __forceinline__ void batch_tile(const BatchDataset& d, uint32_t t, double* __restrict__ tileSums) {
+__device__ __forceinline__ void batch_tile(const BatchDataset& d, uint32_t t, double* __restrict__ tileSums,
+                                           const double* lnPhiTab) {
     const uint32_t code = d.tiles[t];
     const int I = (int)(code >> 16), J = (int)(code & 0xffffu);
     const Real* base = (const Real*)d.delta + (int64_t)t * kTileElems;
     const Real* x = (const Real*)d.x;
-    const Real tau = (Real)d.tau;
-    const Real halfTau = (Real)(0.5 * d.tau);
-    const Real sqrtTau = (Real)sqrt(d.tau);
-    const Real cg = (Real)(kInvSqrt2Pi / sqrt(d.tau));
+    const PairConst<Real> pc = make_pair_const<Real>(d.tau, lnPhiTab);
     double* sum = tileSums + d.tileSumBase + t;
     if (d.truncated) {
         if (d.symmetric)
-            tile_body<Real, P, true, true, true, kGroup>(base, d.planeStride, x, I, J, d.nb, tau, halfTau, sqrtTau,
-                                                         cg, sum, d.part);
+            tile_body<Real, P, true, true, true, kGroup>(base, d.planeStride, x, I, J, d.nb, pc, sum, d.part);
         else
-            tile_body<Real, P, true, false, true, kGroup>(base, d.planeStride, x, I, J, d.nb, tau, halfTau,
-                                                          sqrtTau, cg, sum, d.part);
+            tile_body<Real, P, true, false, true, kGroup>(base, d.planeStride, x, I, J, d.nb, pc, sum, d.part);
     } else {
         if (d.symmetric)
-            tile_body<Real, P, false, true, true, kGroup>(base, d.planeStride, x, I, J, d.nb, tau, halfTau,
-                                                          sqrtTau, cg, sum, d.part);
+            tile_body<Real, P, false, true, true, kGroup>(base, d.planeStride, x, I, J, d.nb, pc, sum, d.part);
         else
-            tile_body<Real, P, false, false, true, kGroup>(base, d.planeStride, x, I, J, d.nb, tau, halfTau,
-                                                           sqrtTau, cg, sum, d.part);
+            tile_body<Real, P, false, false, true, kGroup>(base, d.planeStride, x, I, J, d.nb, pc, sum, d.part);
     }
 }
 
 template <typename Real>
-__device__ __forceinline__ void batch_tile_any(const BatchDataset& d, uint32_t t, double* __restrict__ tileSums) {
+__device__ __forceinline__ void batch_tile_any(const BatchDataset& d, uint32_t t, double* __restrict__ tileSums,
+                                               const double* lnPhiTab) {
     switch (d.kdim) {
-        case 1: batch_tile<Real, 1>(d, t, tileSums); break;
-        case 2: batch_tile<Real, 2>(d, t, tileSums); break;
-        case 3: batch_tile<Real, 3>(d, t, tileSums); break;
-        case 4: batch_tile<Real, 4>(d, t, tileSums); break;
-        case 5: batch_tile<Real, 5>(d, t, tileSums); break;
-        case 6: batch_tile<Real, 6>(d, t, tileSums); break;
-        case 8: batch_tile<Real, 8>(d, t, tileSums); break;
-        case 10: batch_tile<Real, 10>(d, t, tileSums); break;
+        case 1: batch_tile<Real, 1>(d, t, tileSums, lnPhiTab); break;
+        case 2: batch_tile<Real, 2>(d, t, tileSums, lnPhiTab); break;
+        case 3: batch_tile<Real, 3>(d, t, tileSums, lnPhiTab); break;
+        case 4: batch_tile<Real, 4>(d, t, tileSums, lnPhiTab); break;
+        case 5: batch_tile<Real, 5>(d, t, tileSums, lnPhiTab); break;
+        case 6: batch_tile<Real, 6>(d, t, tileSums, lnPhiTab); break;
+        case 8: batch_tile<Real, 8>(d, t, tileSums, lnPhiTab); break;
+        case 10: batch_tile<Real, 10>(d, t, tileSums, lnPhiTab); break;
         default: break;  // the host only admits kernel dimensions <= 10 here
     }
 }
@@ -113,6 +108,9 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batch_tile_kernel(const Batc
                                                                        const BatchItem* __restrict__ items,
                                                                        int numItems,
                                                                        double* __restrict__ tileSums) {
+    constexpr bool kTable = sizeof(Real) == sizeof(double);  // FP64 truncated forms read the table
+    __shared__ __align__(16) double lnPhiTab[kTable ? coeffs::kLnPhiTabRows * coeffs::kLnPhiTabStride : 1];
+    if constexpr (kTable) load_lnphi_table(lnPhiTab);
     const int wavesPerBlock = blockDim.x / kWave;
     const int stride = gridDim.x * wavesPerBlock;
     for (int w = __builtin_amdgcn_readfirstlane(blockIdx.x * wavesPerBlock + threadIdx.x / kWave); w < numItems;
@@ -121,9 +119,9 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batch_tile_kernel(const Batc
         const BatchDataset d = ds[it.ds];
         for (uint32_t c = 0; c < it.count; ++c) {
             if constexpr (KDIM == 0)
-                batch_tile_any<Real>(d, it.tile + c, tileSums);
+                batch_tile_any<Real>(d, it.tile + c, tileSums, lnPhiTab);
             else
-                batch_tile<Real, KDIM>(d, it.tile + c, tileSums);
+                batch_tile<Real, KDIM>(d, it.tile + c, tileSums, lnPhiTab);
         }
     }
 }
@@ -545,6 +543,7 @@ struct Batch {
         planDirty = true;
         if (cachePath) {
             std::ofstream out(cachePath, std::ios_base::app);
+            out.precision(17);  // the time reads back exactly
             out << sig << " " << cfg.wavesPerBlock << " " << cfg.blocksPerCU << " " << cfg.tilesPerItem << " "
                 << cfg.strategy << " " << bestMs << "\n";
         }

@@ -32,6 +32,20 @@ __device__ __forceinline__ double dpp_mov(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// Lane permutations that write every lane (full-wave rotations): no lane
+// keeps its old value, so none needs initialising.
+template <int CTRL>
+__device__ __forceinline__ float dpp_rotate(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_rotate(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
 // Sum over the 64 lanes of a fully active wavefront.  Lane 63 receives the
 // total (other lanes hold partial sums).  Fixed order, so bitwise
 // reproducible.  Lanes masked off by the row masks receive 0 and keep their
@@ -99,7 +113,7 @@ template <>
 struct ColSum<double> {
     double s = 0.0;
     __device__ __forceinline__ void add(double v) { s += v; }
-    __device__ __forceinline__ void rotate() { s = dpp_mov<kDppWaveRol1, 0xf>(s); }
+    __device__ __forceinline__ void rotate() { s = dpp_rotate<kDppWaveRol1>(s); }
     __device__ __forceinline__ double value() const { return s; }
 };
 
@@ -108,8 +122,8 @@ struct ColSum<float> {
     LaneSum<float> k;
     __device__ __forceinline__ void add(float v) { k.add(v); }
     __device__ __forceinline__ void rotate() {
-        k.s = dpp_mov<kDppWaveRol1, 0xf>(k.s);
-        k.c = dpp_mov<kDppWaveRol1, 0xf>(k.c);
+        k.s = dpp_rotate<kDppWaveRol1>(k.s);
+        k.c = dpp_rotate<kDppWaveRol1>(k.c);
     }
     __device__ __forceinline__ double value() const { return k.value(); }
 };

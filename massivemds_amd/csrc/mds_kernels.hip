@@ -21,26 +21,24 @@
 
 namespace mdship {
 
-// Single instance: one wavefront per tile of the upper block triangle.
+// Single instance: one wavefront per tile of the upper block triangle.  The
+// FP64 truncated forms first copy the log Phi table into LDS (every wavefront
+// of the block takes part, before any leaves).
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
     const Real* __restrict__ delta, const Real* __restrict__ x,
     const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
-    Real tau, Real halfTau, Real sqrtTau, Real cg, double* __restrict__ tileSums,
-    double* __restrict__ part, const double* __restrict__ devTau) {
+    double tau, double* __restrict__ tileSums, double* __restrict__ part, const double* __restrict__ devTau) {
+    constexpr bool kTable = TRUNC && sizeof(Real) == sizeof(double);
+    __shared__ __align__(16) double lnPhiTab[kTable ? coeffs::kLnPhiTabRows * coeffs::kLnPhiTabStride : 1];
+    if constexpr (kTable) load_lnphi_table(lnPhiTab);
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
     if (t >= numTiles) return;
-    if (devTau) {  // same arithmetic as the host-side derivation in go()
-        const double v = *devTau;
-        tau = (Real)v;
-        halfTau = (Real)(0.5 * v);
-        sqrtTau = (Real)sqrt(v);
-        cg = (Real)(kInvSqrt2Pi / sqrt(v));
-    }
+    const PairConst<Real> pc = make_pair_const<Real>(devTau ? *devTau : tau, lnPhiTab);
     const uint32_t code = tiles[t];
     tile_body<Real, P, TRUNC, SYM, GRAD, kGroup>(delta + (int64_t)t * kTileElems, planeStride, x,
-                                                 (int)(code >> 16), (int)(code & 0xffffu), nb, tau, halfTau,
-                                                 sqrtTau, cg, tileSums + t, part);
+                                                 (int)(code >> 16), (int)(code & 0xffffu), nb, pc, tileSums + t,
+                                                 part);
 }
 
 constexpr int kReduceWaves = 8;
@@ -98,13 +96,9 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 static hipError_t go(const TileLaunch& a, hipStream_t s) {
     const dim3 grid((a.numTiles + kWavesPerBlock - 1) / kWavesPerBlock);
-    const Real tau = (Real)a.tau;
-    const Real halfTau = (Real)(0.5 * a.tau);
-    const Real sqrtTau = (Real)sqrt(a.tau);
-    const Real cg = (Real)(kInvSqrt2Pi / sqrt(a.tau));
     hipLaunchKernelGGL((tile_kernel<Real, P, TRUNC, SYM, GRAD>), grid, dim3(kBlockThreads), 0, s,
-                       (const Real*)a.delta, (const Real*)a.x, a.tiles, a.numTiles, a.nb,
-                       a.planeStride, tau, halfTau, sqrtTau, cg, a.tileSums, a.part, a.devTau);
+                       (const Real*)a.delta, (const Real*)a.x, a.tiles, a.numTiles, a.nb, a.planeStride, a.tau,
+                       a.tileSums, a.part, a.devTau);
     return hipGetLastError();
 }
 

@@ -1,16 +1,28 @@
 // Host build of the kernels' fused truncation arithmetic (mds_trunc_math.hpp)
 // so that its accuracy can be checked on the CPU against high-precision
 // values.  The hardware reciprocal / reciprocal-square-root seeds are replaced
-// by float-precision seeds of the same (~2^-24) accuracy; the Newton steps,
-// polynomials and rational forms are the ones the device runs.
+// by seeds of the same accuracy (FP64: the correctly rounded value cut to 26
+// significant bits, ~2^-26 like v_rcp_f64 / v_rsq_f64; FP32: float
+// operations); the Newton steps, polynomials, tables and rational forms are
+// the ones the device runs.
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 
 #include "mds_trunc_math.hpp"
 
 namespace {
+double cut26(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, sizeof u);
+    u &= ~((uint64_t(1) << 26) - 1);  // keep 26 of the 52 stored fraction bits
+    std::memcpy(&v, &u, sizeof v);
+    return v;
+}
+
 struct SoftSeeds {
-    static double rcp(double x) { return (double)(1.0f / (float)x); }
-    static double rsq(double x) { return (double)(1.0f / std::sqrt((float)x)); }
+    static double rcp(double x) { return cut26(1.0 / x); }
+    static double rsq(double x) { return cut26(1.0 / std::sqrt(x)); }
     static float rcp(float x) { return 1.0f / x; }
     static float rsq(float x) { return 1.0f / std::sqrt(x); }
     static float exp2(float x) { return std::exp2(x); }
@@ -26,10 +38,25 @@ void mds_mathcheck_f64(const double* d2, long long n, double tau, double* d, dou
     const double halfTau = 0.5 * tau, sqrtTau = std::sqrt(tau);
     const double cg = mdship::kInvSqrt2Pi / sqrtTau;
     for (long long k = 0; k < n; ++k) {
-        mdship::distance64<SoftSeeds>(d2[k], d[k], invd[k]);
+        mdship::distance64<SoftSeeds, 2>(d2[k], d[k], invd[k]);
         const mdship::Trunc<double> t = mdship::trunc64<SoftSeeds>(d[k], d2[k], halfTau, sqrtTau, cg);
         logPhi[k] = t.logPhi;
         gradTerm[k] = t.gradTerm;
+    }
+}
+
+// FP64 as the kernels run it: distance with one Newton step, log Phi and
+// phi / (Phi sqrt tau) from the interval table.
+void mds_mathcheck_f64_table(const double* d2, long long n, double tau, double* d, double* invd,
+                             double* logPhi, double* gradTerm) {
+    const double sqrtTau = std::sqrt(tau);
+    const double sT = mdship::coeffs::kLnPhiTabStep * sqrtTau;
+    const double gScale = mdship::coeffs::kLnPhiTabStep / sqrtTau;
+    for (long long k = 0; k < n; ++k) {
+        mdship::distance64<SoftSeeds, 1>(d2[k], d[k], invd[k]);
+        const mdship::Trunc<double> t = mdship::trunc64_table(d[k], sT, mdship::coeffs::kLnPhiTab);
+        logPhi[k] = t.logPhi;
+        gradTerm[k] = t.gradTerm * gScale;
     }
 }
 

@@ -9,22 +9,62 @@ namespace mdship {
 // Steps per prefetch group of the tile walk.
 constexpr int kGroup = 4;
 
+// Per-evaluation constants of the pair arithmetic.  FP64: a = S sqrt(tau)
+// with S = kLnPhiTabStep (z in the units of the log Phi table, which `tab`
+// points to in LDS); FP32: a = sqrt(tau) and b = 1 / (sqrt(2 pi) sqrt(tau))
+// for the rational form.  gmul turns the truncation's gradient term g into
+// tau * (phi / Phi) / sqrt(tau): FP64 g = d log Phi / d(S z), gmul =
+// S sqrt(tau); FP32 g = phi / (Phi sqrt(tau)), gmul = tau.
 template <typename Real>
-__device__ __forceinline__ Trunc<Real> trunc_terms(Real d, Real d2, Real halfTau, Real sqrtTau, Real cg);
+struct PairConst {
+    Real tau, halfTau, a, b, gmul;
+    const double* tab;
+};
+
+__device__ __forceinline__ PairConst<double> pair_const(double tau, const double* tab) {
+    const double sT = coeffs::kLnPhiTabStep * sqrt(tau);
+    return {tau, 0.5 * tau, sT, 0.0, sT, tab};
+}
+
+__device__ __forceinline__ PairConst<float> pair_const(float tau, float halfTau, float sqrtTau, float cg) {
+    return {tau, halfTau, sqrtTau, cg, tau, nullptr};
+}
+
+// Per-evaluation constants from the precision (as the host holds it, or
+// read from a sampler's device parameter block).
+template <typename Real>
+__device__ __forceinline__ PairConst<Real> make_pair_const(double tau, const double* tab);
 
 template <>
-__device__ __forceinline__ Trunc<double> trunc_terms(double d, double d2, double halfTau, double sqrtTau,
-                                                     double cg) {
-    return trunc64<HwSeeds>(d, d2, halfTau, sqrtTau, cg);
+__device__ __forceinline__ PairConst<double> make_pair_const<double>(double tau, const double* tab) {
+    return pair_const(tau, tab);
 }
 
 template <>
-__device__ __forceinline__ Trunc<float> trunc_terms(float d, float d2, float halfTau, float sqrtTau, float cg) {
-    return trunc32<HwSeeds>(d, d2, halfTau, sqrtTau, cg);
+__device__ __forceinline__ PairConst<float> make_pair_const<float>(double tau, const double* tab) {
+    (void)tab;
+    return pair_const((float)tau, (float)(0.5 * tau), (float)sqrt(tau), (float)(kInvSqrt2Pi / sqrt(tau)));
+}
+
+// Copies the log Phi table into LDS with every thread of the block and
+// waits for it; called before any wavefront of the block leaves.
+__device__ __forceinline__ void load_lnphi_table(double* lds) {
+    constexpr int count = coeffs::kLnPhiTabRows * coeffs::kLnPhiTabStride;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) lds[i] = coeffs::kLnPhiTab[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ Trunc<double> trunc_terms(double d, double d2, const PairConst<double>& c) {
+    (void)d2;
+    return trunc64_table(d, c.a, c.tab);
+}
+
+__device__ __forceinline__ Trunc<float> trunc_terms(float d, float d2, const PairConst<float>& c) {
+    return trunc32<HwSeeds>(d, d2, c.halfTau, c.a, c.b);
 }
 
 __device__ __forceinline__ void distance(double d2, double& d, double& invd) {
-    distance64<HwSeeds>(d2, d, invd);
+    distance64<HwSeeds, 1>(d2, d, invd);
 }
 __device__ __forceinline__ void distance(float d2, float& d, float& invd) { distance32<HwSeeds>(d2, d, invd); }
 
@@ -40,8 +80,8 @@ struct TileAcc {
 // with observations dij = delta_ij and dji = delta_ji.
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi)[P], const Real (&xj)[P],
-                                          Real dij, Real dji, bool inTri, Real tau, Real halfTau,
-                                          Real sqrtTau, Real cg) {
+                                          Real dij, Real dji, bool inTri, const PairConst<Real>& pc) {
+    const Real tau = pc.tau, halfTau = pc.halfTau;
     const bool vij = inTri && !m_isnan(dij);
     const bool vji = inTri && !m_isnan(dji);
     Real diff[P];
@@ -59,7 +99,7 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
     Real lij, lji;
     Real gt = Real(0);
     if (TRUNC) {
-        const Trunc<Real> tt = trunc_terms<Real>(d, d2, halfTau, sqrtTau, cg);
+        const Trunc<Real> tt = trunc_terms(d, d2, pc);
         lij = halfTau * rij * rij + tt.logPhi;
         lji = halfTau * rji * rji + tt.logPhi;
         gt = tt.gradTerm;
@@ -67,17 +107,19 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
         lij = rij * rij;
         lji = rji * rji;
     }
+    // symmetric data: each unordered pair counts twice; the factor 2 is
+    // applied to the tile's sum (exact, so the result is the same)
     if (SYM)
-        acc.s.add(vij ? lij + lij : Real(0));
+        acc.s.add(vij ? lij : Real(0));
     else
         acc.s.add((vij ? lij : Real(0)) + (vji ? lji : Real(0)));
 
     if (GRAD) {
         // c = (r - phi / (Phi sqrt tau)) * tau / d for each ordered pair
         // (src/NewMultiDimensionalScaling.hpp:435-444); masked pairs give 0.
-        const Real tauOverD = tau * invd;
-        const Real cij = vij ? (rij - gt) * tauOverD : Real(0);
-        const Real cji = SYM ? cij : (vji ? (rji - gt) * tauOverD : Real(0));
+        const Real cij = vij ? (TRUNC ? fma(-gt, pc.gmul, tau * rij) : tau * rij) * invd : Real(0);
+        const Real cji =
+            SYM ? cij : (vji ? (TRUNC ? fma(-gt, pc.gmul, tau * rji) : tau * rji) * invd : Real(0));
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const Real rowTerm = cij * diff[p];
@@ -99,8 +141,8 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
 // *tileSum (lane 63) and the two gradient slots of part ([nb][nb][P][64]).
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, int G>
 __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int64_t planeStride,
-                                          const Real* __restrict__ x, int I, int J, int nb, Real tau,
-                                          Real halfTau, Real sqrtTau, Real cg, double* __restrict__ tileSum,
+                                          const Real* __restrict__ x, int I, int J, int nb,
+                                          const PairConst<Real>& pc, double* __restrict__ tileSum,
                                           double* __restrict__ part) {
     static_assert(kTile % (2 * G) == 0, "group size must divide half a tile");
     const int lane = threadIdx.x % kWave;
@@ -136,9 +178,9 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
             const int k = k0 + g;
             // diagonal tiles keep i < j only: lane < (lane + k) mod 64
             const bool inTri = !diag || (k > 0 && lane + k < kTile);
-            pair_step<Real, P, TRUNC, SYM, GRAD>(acc, xi, xj, b[g], bT[g], inTri, tau, halfTau, sqrtTau, cg);
+            pair_step<Real, P, TRUNC, SYM, GRAD>(acc, xi, xj, b[g], bT[g], inTri, pc);
 #pragma unroll
-            for (int p = 0; p < P; ++p) xj[p] = dpp_mov<kDppWaveRol1, 0xf>(xj[p]);
+            for (int p = 0; p < P; ++p) xj[p] = dpp_rotate<kDppWaveRol1>(xj[p]);
         }
     };
 
@@ -153,7 +195,7 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
     }
 
     const double s = wave_sum_to_last(acc.s.value());
-    if (lane == kWave - 1) *tileSum = s;
+    if (lane == kWave - 1) *tileSum = SYM ? 2.0 * s : s;
 
     if (GRAD) {
         double* rowSlot = part + ((int64_t)I * nb + J) * (P * kTile) + lane;

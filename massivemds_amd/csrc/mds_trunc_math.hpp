@@ -1,7 +1,16 @@
 // Fused per-pair arithmetic of the truncated-normal MDS terms.
 //
+// FP64 (trunc64_table): log Phi(z) from a table of degree-5 polynomials on
+// intervals of z of width 1/64 (held in LDS by the kernels), and
+// phi(z) / Phi(z) as the same polynomial's derivative, since
+// d/dz log Phi = phi / Phi; the fit is joint on both, to ~1e-16 (log Phi)
+// and ~7e-15 (phi / Phi) absolute.
+//
+// The rational form below (trunc64, trunc32) needs no table; the FP32
+// kernels use it, and the host check keeps the FP64 one as a second,
+// independent route to the same values.
 // For a pair at squared distance d2 and precision tau, with z = d sqrt(tau):
-//   d, 1/d                 from one reciprocal square root (+ Newton steps)
+//   d, 1/d                 from one reciprocal square root (+ Newton step)
 //   e = exp(-z^2 / 2)      own range-reduced polynomial (2^k scaling)
 //   Q(z) = phi(z) m(z)     upper tail through the Mills ratio m = P / A
 //   log Phi(z) = log(1 - Q) = 2 atanh(s),  s = -Q / (2 - Q)
@@ -43,26 +52,28 @@ __host__ __device__ inline T horner(const T (&c)[N], T x) {
     return v;
 }
 
-// d = sqrt(d2) and 1/d.  d2 = 0 gives d = 0 and 1/d = inf, as sqrt and a
-// division would.
-template <class S>
+// d = sqrt(d2) and 1/d.  d2 = 0 gives d = 0 and 1/d = NaN (there the
+// reference's gradient term (r / d) (x_i - x_j) is inf * 0 = NaN as well, and
+// masked pairs are zeroed after); a NaN d2 gives NaN for both.  The seed's relative error e0 (v_rsq_f64: ~1.7e-8) becomes
+// ~1.5 e0^2 after NEWTON = 1 step (~5e-16) and rounding level after 2.
+template <class S, int NEWTON = 1>
 __host__ __device__ inline void distance64(double d2, double& d, double& invd) {
     double y = S::rsq(d2);
     const double h = 0.5 * d2;
 #pragma unroll
-    for (int it = 0; it < 2; ++it) {
+    for (int it = 0; it < NEWTON; ++it) {
         const double hy = h * y;
         y = fma(y, fma(-hy, y, 0.5), y);
     }
-    invd = d2 > 0.0 ? y : __builtin_inf();
-    d = d2 > 0.0 ? d2 * y : 0.0;
+    invd = y;
+    d = d2 == 0.0 ? 0.0 : d2 * y;
 }
 
 template <class S>
 __host__ __device__ inline void distance32(float d2, float& d, float& invd) {
     const float y = S::rsq(d2);
-    invd = d2 > 0.0f ? y : __builtin_inff();
-    d = d2 > 0.0f ? d2 * y : 0.0f;
+    invd = d2 == 0.0f ? __builtin_inff() : y;
+    d = d2 == 0.0f ? 0.0f : d2 * y;
 }
 
 // exp(-w) for w >= 0.
@@ -102,6 +113,40 @@ __host__ __device__ inline Trunc<double> trunc64(double d, double d2, double hal
     t.logPhi = 2.0 * s * horner(coeffs::kAtanh64, s * s);
     t.gradTerm = e * cg * A * invB;
     return t;
+}
+
+// Table form.  sT = kLnPhiTabStep sqrt(tau) (z in table units, zT = sT d);
+// tab: kLnPhiTabRows rows of kLnPhiTabStride values, 16-byte aligned (LDS on
+// the device), each row read as 16-byte pairs.  Returns log Phi(z) and, as
+// gradTerm, its derivative in table units, d log Phi / d zT =
+// phi(z) / (kLnPhiTabStep Phi(z)).  zT is clamped to the zero row, which also
+// catches inf and NaN distances.
+__host__ __device__ inline Trunc<double> trunc64_table(double d, double sT, const double* __restrict__ tab) {
+    constexpr int kPairs = coeffs::kLnPhiTabStride / 2;
+    static_assert(coeffs::kLnPhiTabStride % 2 == 0 && coeffs::kLnPhiTabDeg < coeffs::kLnPhiTabStride,
+                  "rows are read as 16-byte pairs");
+    const double zT = fmin(d * sT, coeffs::kLnPhiTabClamp);
+    const double fl = floor(zT);
+    const double t = zT - fl;
+    const double2* row = reinterpret_cast<const double2*>(tab) + (int)(fl * kPairs);  // exact: fl < 2^20
+    double c[coeffs::kLnPhiTabStride];
+#pragma unroll
+    for (int h = 0; h < kPairs; ++h) {
+        const double2 v = row[h];
+        c[2 * h] = v.x;
+        c[2 * h + 1] = v.y;
+    }
+    double p = c[coeffs::kLnPhiTabDeg];
+    double q = 0.0;
+#pragma unroll
+    for (int j = coeffs::kLnPhiTabDeg - 1; j >= 0; --j) {
+        q = fma(q, t, p);
+        p = fma(p, t, c[j]);
+    }
+    Trunc<double> r;
+    r.logPhi = p;
+    r.gradTerm = q;
+    return r;
 }
 
 template <class S>

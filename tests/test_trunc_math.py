@@ -24,18 +24,18 @@ def _lib():
     if not hipbuild.MATHCHECK_LIB.exists():
         hipbuild.build()
     lib = ctypes.CDLL(str(hipbuild.MATHCHECK_LIB))
-    for name in ("mds_mathcheck_f64", "mds_mathcheck_f32"):
+    for name in ("mds_mathcheck_f64", "mds_mathcheck_f64_table", "mds_mathcheck_f32"):
         fn = getattr(lib, name)
         fn.argtypes = [_dp, ctypes.c_longlong, ctypes.c_double, _dp, _dp, _dp, _dp]
         fn.restype = None
     return lib
 
 
-def _eval(fp32, d2, tau):
+def _eval(fp32, d2, tau, table=False):
     lib = _lib()
     d2 = np.ascontiguousarray(d2, dtype=np.float64)
     outs = [np.empty_like(d2) for _ in range(4)]
-    fn = lib.mds_mathcheck_f32 if fp32 else lib.mds_mathcheck_f64
+    fn = lib.mds_mathcheck_f32 if fp32 else (lib.mds_mathcheck_f64_table if table else lib.mds_mathcheck_f64)
     fn(d2.ctypes.data_as(_dp), d2.size, tau, *[o.ctypes.data_as(_dp) for o in outs])
     return outs
 
@@ -74,11 +74,46 @@ def test_fp64_terms(tau):
         assert abs(grad[k] - egrad) <= (1e-15 + 3e-16 * w) * egrad + 1e-290, (v, grad[k], egrad)
 
 
+@pytest.mark.parametrize("tau", [0.5, 1.0, 2.0, 3.7])
+def test_fp64_table_terms(tau):
+    """The FP64 kernels' form: one Newton step for d and 1/d (relative error
+    ~1.5 e0^2 from the ~2^-26 seed), log Phi and phi / Phi from the interval
+    table.  Both table terms are checked in absolute error: they enter
+    O(1) sums (log Phi) and the difference r - phi / (Phi sqrt tau)."""
+    d2 = _grid(tau)
+    d, invd, log_phi, grad = _eval(False, d2, tau, table=True)
+    for k, v in enumerate(d2):
+        ed, elog, egrad = _exact(v, tau)
+        assert abs(d[k] - ed) <= 8e-16 * ed, (v, d[k], ed)
+        if v > 0:
+            assert abs(invd[k] * ed - 1) <= 8e-16
+        assert abs(log_phi[k] - elog) <= 4e-16, (v, log_phi[k], elog)
+        assert abs(grad[k] - egrad) <= 2e-14 / np.sqrt(tau), (v, grad[k], egrad)
+
+
+def test_fp64_table_edges():
+    """Interval edges (t = 0 of every row), the clamp to the zero row, and
+    distances that are not finite."""
+    tau = 2.0
+    z = np.concatenate([np.arange(0, 641) / 64.0, [10.0 - 1e-12, 10.0, 10.0 + 1e-12, 1e3]])
+    d2 = (z / np.sqrt(tau)) ** 2
+    d, invd, log_phi, grad = _eval(False, d2, tau, table=True)
+    for k, v in enumerate(d2):
+        _, elog, egrad = _exact(v, tau)
+        assert abs(log_phi[k] - elog) <= 4e-16 and abs(grad[k] - egrad) <= 2e-14, (z[k], log_phi[k], elog)
+    d, invd, log_phi, grad = _eval(False, np.array([np.inf, np.nan]), tau, table=True)
+    assert np.all(log_phi == 0.0) and np.all(grad == 0.0)
+    assert np.isnan(d[1]) and np.isnan(invd[1])  # NaN propagates through d (and so the residual)
+
+
 def test_fp64_zero_distance():
-    d, invd, log_phi, grad = _eval(False, np.array([0.0]), 2.0)
-    assert d[0] == 0.0 and np.isinf(invd[0])
-    assert abs(log_phi[0] - np.log(0.5)) <= 5e-16
-    assert abs(grad[0] - 2 * 0.3989422804014327 / np.sqrt(2.0)) <= 1e-15
+    for table in (False, True):
+        d, invd, log_phi, grad = _eval(False, np.array([0.0]), 2.0, table=table)
+        # 1/d is NaN at d = 0: the gradient term (r / d) (x_i - x_j) is NaN
+        # there in the reference too (inf * 0)
+        assert d[0] == 0.0 and np.isnan(invd[0])
+        assert abs(log_phi[0] - np.log(0.5)) <= 5e-16
+        assert abs(grad[0] - 2 * 0.3989422804014327 / np.sqrt(2.0)) <= (2e-14 if table else 1e-15)
 
 
 @pytest.mark.parametrize("tau", [0.5, 2.0])
