@@ -65,7 +65,7 @@ constexpr int kBatchMaxThreads = 512;
 
 template <typename Real, int P>
 __device__ __forceinline__ void batch_tile(const BatchDataset& d, uint32_t t, double* __restrict__ tileSums,
-                                           const double* lnPhiTab) {
+                                           const Real* lnPhiTab) {
     const uint32_t code = d.tiles[t];
     const int I = (int)(code >> 16), J = (int)(code & 0xffffu);
     const Real* base = (const Real*)d.delta + (int64_t)t * kTileElems;
@@ -87,7 +87,7 @@ __device__ __forceinline__ void batch_tile(const BatchDataset& d, uint32_t t, do
 
 template <typename Real>
 __device__ __forceinline__ void batch_tile_any(const BatchDataset& d, uint32_t t, double* __restrict__ tileSums,
-                                               const double* lnPhiTab) {
+                                               const Real* lnPhiTab) {
     switch (d.kdim) {
         case 1: batch_tile<Real, 1>(d, t, tileSums, lnPhiTab); break;
         case 2: batch_tile<Real, 2>(d, t, tileSums, lnPhiTab); break;
@@ -108,9 +108,8 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batch_tile_kernel(const Batc
                                                                        const BatchItem* __restrict__ items,
                                                                        int numItems,
                                                                        double* __restrict__ tileSums) {
-    constexpr bool kTable = sizeof(Real) == sizeof(double);  // FP64 truncated forms read the table
-    __shared__ __align__(16) double lnPhiTab[kTable ? coeffs::kLnPhiTabRows * coeffs::kLnPhiTabStride : 1];
-    if constexpr (kTable) load_lnphi_table(lnPhiTab);
+    __shared__ __align__(16) Real lnPhiTab[TableOf<Real>::kCount];  // truncated datasets read it
+    load_lnphi_table(lnPhiTab);
     const int wavesPerBlock = blockDim.x / kWave;
     const int stride = gridDim.x * wavesPerBlock;
     for (int w = __builtin_amdgcn_readfirstlane(blockIdx.x * wavesPerBlock + threadIdx.x / kWave); w < numItems;

@@ -9,6 +9,13 @@
 #include "mds_internal.hpp"
 #include "mds_trunc_math.hpp"
 
+// Floating-point contraction only within one expression: whether a product
+// is fused into a later sum must not depend on how a function was inlined
+// (the kernels' results are bitwise independent of the launch shape, and
+// different instantiations share this code).  Fused products the kernels
+// rely on are written as fma().
+#pragma clang fp contract(on)
+
 namespace mdship {
 
 // DPP control words (GFX9 encoding, valid on gfx950).
@@ -76,10 +83,14 @@ __device__ __forceinline__ double read_last_lane(double v) {
 template <typename Real>
 struct LaneSum;
 
+// add(v) adds a value; add_product(a, b) adds a * b with one rounding (an
+// explicit fma, so that the result never depends on whether the compiler
+// fuses a product into the sum).
 template <>
 struct LaneSum<double> {
     double s = 0.0;
     __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ void add_product(double a, double b) { s = fma(a, b, s); }
     __device__ __forceinline__ double value() const { return s; }
 };
 
@@ -87,13 +98,17 @@ template <>
 struct LaneSum<float> {
     float s = 0.0f;
     float c = 0.0f;
-    __device__ __forceinline__ void add(float v) {
-        const float y = v - c;
+    __device__ __forceinline__ void add(float v) { add_compensated(v - c); }
+    __device__ __forceinline__ void add_product(float a, float b) { add_compensated(fmaf(a, b, -c)); }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+
+  private:
+    // y = v - c, the value corrected by the running compensation
+    __device__ __forceinline__ void add_compensated(float y) {
         const float t = s + y;
         c = (t - s) - y;
         s = t;
     }
-    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
 __device__ __forceinline__ bool m_isnan(double v) { return v != v; }
@@ -113,6 +128,7 @@ template <>
 struct ColSum<double> {
     double s = 0.0;
     __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ void add_product(double a, double b) { s = fma(a, b, s); }
     __device__ __forceinline__ void rotate() { s = dpp_rotate<kDppWaveRol1>(s); }
     __device__ __forceinline__ double value() const { return s; }
 };
@@ -121,6 +137,7 @@ template <>
 struct ColSum<float> {
     LaneSum<float> k;
     __device__ __forceinline__ void add(float v) { k.add(v); }
+    __device__ __forceinline__ void add_product(float a, float b) { k.add_product(a, b); }
     __device__ __forceinline__ void rotate() {
         k.s = dpp_rotate<kDppWaveRol1>(k.s);
         k.c = dpp_rotate<kDppWaveRol1>(k.c);

@@ -22,16 +22,15 @@
 namespace mdship {
 
 // Single instance: one wavefront per tile of the upper block triangle.  The
-// FP64 truncated forms first copy the log Phi table into LDS (every wavefront
-// of the block takes part, before any leaves).
+// truncated forms first copy the log Phi table into LDS (every wavefront of
+// the block takes part, before any leaves).
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
     const Real* __restrict__ delta, const Real* __restrict__ x,
     const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
     double tau, double* __restrict__ tileSums, double* __restrict__ part, const double* __restrict__ devTau) {
-    constexpr bool kTable = TRUNC && sizeof(Real) == sizeof(double);
-    __shared__ __align__(16) double lnPhiTab[kTable ? coeffs::kLnPhiTabRows * coeffs::kLnPhiTabStride : 1];
-    if constexpr (kTable) load_lnphi_table(lnPhiTab);
+    __shared__ __align__(16) Real lnPhiTab[TRUNC ? TableOf<Real>::kCount : 1];
+    if constexpr (TRUNC) load_lnphi_table(lnPhiTab);
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
     if (t >= numTiles) return;
     const PairConst<Real> pc = make_pair_const<Real>(devTau ? *devTau : tau, lnPhiTab);

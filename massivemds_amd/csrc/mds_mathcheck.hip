@@ -1,10 +1,9 @@
 // Host build of the kernels' fused truncation arithmetic (mds_trunc_math.hpp)
 // so that its accuracy can be checked on the CPU against high-precision
-// values.  The hardware reciprocal / reciprocal-square-root seeds are replaced
-// by seeds of the same accuracy (FP64: the correctly rounded value cut to 26
-// significant bits, ~2^-26 like v_rcp_f64 / v_rsq_f64; FP32: float
-// operations); the Newton steps, polynomials, tables and rational forms are
-// the ones the device runs.
+// values.  The hardware reciprocal-square-root seeds are replaced by seeds of
+// the same accuracy (FP64: the correctly rounded value cut to 26 significant
+// bits, ~2^-26 like v_rsq_f64; FP32: float operations); the Newton step and
+// the log Phi tables are the ones the device runs.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -21,34 +20,17 @@ double cut26(double v) {
 }
 
 struct SoftSeeds {
-    static double rcp(double x) { return cut26(1.0 / x); }
     static double rsq(double x) { return cut26(1.0 / std::sqrt(x)); }
-    static float rcp(float x) { return 1.0f / x; }
     static float rsq(float x) { return 1.0f / std::sqrt(x); }
-    static float exp2(float x) { return std::exp2(x); }
 };
 }  // namespace
 
 extern "C" {
 
 // For each squared distance d2[k]: d, 1/d, log Phi(d sqrt tau) and
-// phi / (Phi sqrt tau), FP64 forms.
-void mds_mathcheck_f64(const double* d2, long long n, double tau, double* d, double* invd,
-                       double* logPhi, double* gradTerm) {
-    const double halfTau = 0.5 * tau, sqrtTau = std::sqrt(tau);
-    const double cg = mdship::kInvSqrt2Pi / sqrtTau;
-    for (long long k = 0; k < n; ++k) {
-        mdship::distance64<SoftSeeds, 2>(d2[k], d[k], invd[k]);
-        const mdship::Trunc<double> t = mdship::trunc64<SoftSeeds>(d[k], d2[k], halfTau, sqrtTau, cg);
-        logPhi[k] = t.logPhi;
-        gradTerm[k] = t.gradTerm;
-    }
-}
-
-// FP64 as the kernels run it: distance with one Newton step, log Phi and
-// phi / (Phi sqrt tau) from the interval table.
-void mds_mathcheck_f64_table(const double* d2, long long n, double tau, double* d, double* invd,
-                             double* logPhi, double* gradTerm) {
+// phi / (Phi sqrt tau), as the FP64 kernels compute them.
+void mds_mathcheck_f64(const double* d2, long long n, double tau, double* d, double* invd, double* logPhi,
+                       double* gradTerm) {
     const double sqrtTau = std::sqrt(tau);
     const double sT = mdship::coeffs::kLnPhiTabStep * sqrtTau;
     const double gScale = mdship::coeffs::kLnPhiTabStep / sqrtTau;
@@ -61,19 +43,19 @@ void mds_mathcheck_f64_table(const double* d2, long long n, double tau, double* 
 }
 
 // FP32 forms, results widened to double.
-void mds_mathcheck_f32(const double* d2, long long n, double tau, double* d, double* invd,
-                       double* logPhi, double* gradTerm) {
-    const float halfTau = (float)(0.5 * tau), sqrtTau = (float)std::sqrt(tau);
-    const float cg = (float)(mdship::kInvSqrt2Pi / std::sqrt(tau));
+void mds_mathcheck_f32(const double* d2, long long n, double tau, double* d, double* invd, double* logPhi,
+                       double* gradTerm) {
+    const float sT = (float)(mdship::coeffs::kLnPhiTab32Step * std::sqrt(tau));
+    const double gScale = mdship::coeffs::kLnPhiTab32Step / std::sqrt(tau);
     for (long long k = 0; k < n; ++k) {
         float df, idf;
         const float d2f = (float)d2[k];
         mdship::distance32<SoftSeeds>(d2f, df, idf);
-        const mdship::Trunc<float> t = mdship::trunc32<SoftSeeds>(df, d2f, halfTau, sqrtTau, cg);
+        const mdship::Trunc<float> t = mdship::trunc32_table(df, sT, mdship::coeffs::kLnPhiTab32);
         d[k] = df;
         invd[k] = idf;
         logPhi[k] = t.logPhi;
-        gradTerm[k] = t.gradTerm;
+        gradTerm[k] = t.gradTerm * gScale;
     }
 }
 }

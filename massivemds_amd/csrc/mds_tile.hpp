@@ -9,58 +9,54 @@ namespace mdship {
 // Steps per prefetch group of the tile walk.
 constexpr int kGroup = 4;
 
-// Per-evaluation constants of the pair arithmetic.  FP64: a = S sqrt(tau)
-// with S = kLnPhiTabStep (z in the units of the log Phi table, which `tab`
-// points to in LDS); FP32: a = sqrt(tau) and b = 1 / (sqrt(2 pi) sqrt(tau))
-// for the rational form.  gmul turns the truncation's gradient term g into
-// tau * (phi / Phi) / sqrt(tau): FP64 g = d log Phi / d(S z), gmul =
-// S sqrt(tau); FP32 g = phi / (Phi sqrt(tau)), gmul = tau.
+// Per-evaluation constants of the pair arithmetic: tau, tau / 2, and
+// sT = S sqrt(tau), z in the units of the log Phi table (S = 64 for FP64,
+// 32 for FP32), which `tab` points to in LDS.  The truncation's gradient
+// term g = d log Phi / d(S z) enters the pair coefficient as
+// tau (phi / Phi) / sqrt(tau) = sT g.
 template <typename Real>
 struct PairConst {
-    Real tau, halfTau, a, b, gmul;
-    const double* tab;
+    Real tau, halfTau, sT;
+    const Real* tab;
 };
 
-__device__ __forceinline__ PairConst<double> pair_const(double tau, const double* tab) {
-    const double sT = coeffs::kLnPhiTabStep * sqrt(tau);
-    return {tau, 0.5 * tau, sT, 0.0, sT, tab};
-}
-
-__device__ __forceinline__ PairConst<float> pair_const(float tau, float halfTau, float sqrtTau, float cg) {
-    return {tau, halfTau, sqrtTau, cg, tau, nullptr};
-}
+template <typename Real>
+struct TableOf;
+template <>
+struct TableOf<double> {
+    static constexpr int kStep = coeffs::kLnPhiTabStep;
+    static constexpr int kCount = coeffs::kLnPhiTabRows * coeffs::kLnPhiTabStride;
+    __device__ static double value(int i) { return coeffs::kLnPhiTab[i]; }
+};
+template <>
+struct TableOf<float> {
+    static constexpr int kStep = coeffs::kLnPhiTab32Step;
+    static constexpr int kCount = coeffs::kLnPhiTab32Rows * coeffs::kLnPhiTab32Stride;
+    __device__ static float value(int i) { return coeffs::kLnPhiTab32[i]; }
+};
 
 // Per-evaluation constants from the precision (as the host holds it, or
-// read from a sampler's device parameter block).
+// read from a sampler's device parameter block), computed in FP64.
 template <typename Real>
-__device__ __forceinline__ PairConst<Real> make_pair_const(double tau, const double* tab);
-
-template <>
-__device__ __forceinline__ PairConst<double> make_pair_const<double>(double tau, const double* tab) {
-    return pair_const(tau, tab);
+__device__ __forceinline__ PairConst<Real> make_pair_const(double tau, const Real* tab) {
+    return {(Real)tau, (Real)(0.5 * tau), (Real)(TableOf<Real>::kStep * sqrt(tau)), tab};
 }
 
-template <>
-__device__ __forceinline__ PairConst<float> make_pair_const<float>(double tau, const double* tab) {
-    (void)tab;
-    return pair_const((float)tau, (float)(0.5 * tau), (float)sqrt(tau), (float)(kInvSqrt2Pi / sqrt(tau)));
-}
-
-// Copies the log Phi table into LDS with every thread of the block and
-// waits for it; called before any wavefront of the block leaves.
-__device__ __forceinline__ void load_lnphi_table(double* lds) {
-    constexpr int count = coeffs::kLnPhiTabRows * coeffs::kLnPhiTabStride;
-    for (int i = threadIdx.x; i < count; i += blockDim.x) lds[i] = coeffs::kLnPhiTab[i];
+// Copies the log Phi table of this precision into LDS with every thread of
+// the block and waits for it; called before any wavefront of the block
+// leaves.
+template <typename Real>
+__device__ __forceinline__ void load_lnphi_table(Real* lds) {
+    for (int i = threadIdx.x; i < TableOf<Real>::kCount; i += blockDim.x) lds[i] = TableOf<Real>::value(i);
     __syncthreads();
 }
 
-__device__ __forceinline__ Trunc<double> trunc_terms(double d, double d2, const PairConst<double>& c) {
-    (void)d2;
-    return trunc64_table(d, c.a, c.tab);
+__device__ __forceinline__ Trunc<double> trunc_terms(double d, const PairConst<double>& c) {
+    return trunc64_table(d, c.sT, c.tab);
 }
 
-__device__ __forceinline__ Trunc<float> trunc_terms(float d, float d2, const PairConst<float>& c) {
-    return trunc32<HwSeeds>(d, d2, c.halfTau, c.a, c.b);
+__device__ __forceinline__ Trunc<float> trunc_terms(float d, const PairConst<float>& c) {
+    return trunc32_table(d, c.sT, c.tab);
 }
 
 __device__ __forceinline__ void distance(double d2, double& d, double& invd) {
@@ -99,7 +95,7 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
     Real lij, lji;
     Real gt = Real(0);
     if (TRUNC) {
-        const Trunc<Real> tt = trunc_terms(d, d2, pc);
+        const Trunc<Real> tt = trunc_terms(d, pc);
         lij = halfTau * rij * rij + tt.logPhi;
         lji = halfTau * rji * rji + tt.logPhi;
         gt = tt.gradTerm;
@@ -117,14 +113,13 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
     if (GRAD) {
         // c = (r - phi / (Phi sqrt tau)) * tau / d for each ordered pair
         // (src/NewMultiDimensionalScaling.hpp:435-444); masked pairs give 0.
-        const Real cij = vij ? (TRUNC ? fma(-gt, pc.gmul, tau * rij) : tau * rij) * invd : Real(0);
+        const Real cij = vij ? (TRUNC ? fma(-gt, pc.sT, tau * rij) : tau * rij) * invd : Real(0);
         const Real cji =
-            SYM ? cij : (vji ? (TRUNC ? fma(-gt, pc.gmul, tau * rji) : tau * rji) * invd : Real(0));
+            SYM ? cij : (vji ? (TRUNC ? fma(-gt, pc.sT, tau * rji) : tau * rji) * invd : Real(0));
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const Real rowTerm = cij * diff[p];
-            acc.row[p].add(rowTerm);
-            acc.col[p].add(SYM ? -rowTerm : -(cji * diff[p]));
+            acc.row[p].add_product(cij, diff[p]);
+            acc.col[p].add_product(SYM ? -cij : -cji, diff[p]);
             acc.col[p].rotate();
         }
     }
