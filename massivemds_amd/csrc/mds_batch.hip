@@ -134,9 +134,21 @@ __global__ __launch_bounds__(kBatchReduceWaves * kWave) void batch_reduce_kernel
     const ReduceItem item = items[blockIdx.x];
     const BatchDataset d = ds[item.ds];
     const int tid = threadIdx.x;
+    // loads issued kAhead iterations at a time, added in the plain loop's order
+    constexpr int kAhead = 4;
+    constexpr int kThreads = kBatchReduceWaves * kWave;
     if (item.K < 0) {
         double acc = 0.0;
-        for (int k = tid; k < d.numTiles; k += kBatchReduceWaves * kWave) acc += tileSums[d.tileSumBase + k];
+        const double* ts = tileSums + d.tileSumBase;
+        int k = tid;
+        for (; k + (kAhead - 1) * kThreads < d.numTiles; k += kAhead * kThreads) {
+            double v[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) v[u] = ts[k + u * kThreads];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) acc += v[u];
+        }
+        for (; k < d.numTiles; k += kThreads) acc += ts[k];
         red[tid] = acc;
         __syncthreads();
         for (int s = kBatchReduceWaves * kWave / 2; s > 0; s >>= 1) {
@@ -151,8 +163,17 @@ __global__ __launch_bounds__(kBatchReduceWaves * kWave) void batch_reduce_kernel
     const int row = K * kTile + lane;
     for (int p = 0; p < d.kdim; ++p) {
         double acc = 0.0;
-        for (int J = wave; J < d.nb; J += kBatchReduceWaves)
-            acc += d.part[(((int64_t)K * d.nb + J) * d.kdim + p) * kTile + lane];
+        const double* col = d.part + ((int64_t)K * d.nb * d.kdim + p) * kTile + lane;  // + J * kdim * 64
+        const int64_t stride = (int64_t)d.kdim * kTile;
+        int J = wave;
+        for (; J + (kAhead - 1) * kBatchReduceWaves < d.nb; J += kAhead * kBatchReduceWaves) {
+            double v[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) v[u] = col[(J + u * kBatchReduceWaves) * stride];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) acc += v[u];
+        }
+        for (; J < d.nb; J += kBatchReduceWaves) acc += col[J * stride];
         red[tid] = acc;
         __syncthreads();
         if (wave == 0) {

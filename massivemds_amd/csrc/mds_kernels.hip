@@ -53,9 +53,20 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
     int nb, int n, int pOut, double* __restrict__ grad, double* __restrict__ sumOut) {
     __shared__ double red[kReduceThreads * (P > 1 ? P : 1)];
     const int tid = threadIdx.x;
+    // Loads are issued kAhead iterations at a time and added in the same
+    // order as a plain loop would (latency hidden, results unchanged).
+    constexpr int kAhead = 4;
     if (blockIdx.x == gridDim.x - 1) {
         double acc = 0.0;
-        for (int k = tid; k < numTiles; k += kReduceThreads) acc += tileSums[k];
+        int k = tid;
+        for (; k + (2 * kAhead - 1) * kReduceThreads < numTiles; k += 2 * kAhead * kReduceThreads) {
+            double v[2 * kAhead];
+#pragma unroll
+            for (int u = 0; u < 2 * kAhead; ++u) v[u] = tileSums[k + u * kReduceThreads];
+#pragma unroll
+            for (int u = 0; u < 2 * kAhead; ++u) acc += v[u];
+        }
+        for (; k < numTiles; k += kReduceThreads) acc += tileSums[k];
         red[tid] = acc;
         __syncthreads();
         for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
@@ -72,7 +83,21 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
 #pragma unroll
     for (int p = 0; p < P; ++p) acc[p] = 0.0;
     const double* base = part + (int64_t)K * nb * (P * kTile) + lane;
-    for (int J = wave; J < nb; J += kReduceWaves) {
+    int J = wave;
+    for (; J + (kAhead - 1) * kReduceWaves < nb; J += kAhead * kReduceWaves) {
+        double v[kAhead][P];
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            const double* slot = base + (int64_t)(J + u * kReduceWaves) * (P * kTile);
+#pragma unroll
+            for (int p = 0; p < P; ++p) v[u][p] = slot[p * kTile];
+        }
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u)
+#pragma unroll
+            for (int p = 0; p < P; ++p) acc[p] += v[u][p];
+    }
+    for (; J < nb; J += kReduceWaves) {
         const double* slot = base + (int64_t)J * (P * kTile);
 #pragma unroll
         for (int p = 0; p < P; ++p) acc[p] += slot[p * kTile];
