@@ -100,34 +100,91 @@ def _run_gpu(n, p, tau, delta, x, fp32, steps, warmup):
     }
 
 
+def _cpu_host():
+    """The host the CPU baselines ran on (model, sockets, logical CPUs, SMT)."""
+    model, sockets, siblings, cores = None, set(), None, None
+    try:
+        for line in open("/proc/cpuinfo"):
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key == "model name" and model is None:
+                model = val
+            elif key == "physical id":
+                sockets.add(val)
+            elif key == "siblings" and siblings is None:
+                siblings = int(val)
+            elif key == "cpu cores" and cores is None:
+                cores = int(val)
+    except OSError:
+        pass
+    smt = None
+    try:
+        smt = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        if siblings and cores:
+            smt = siblings > cores
+    return {"model": model, "sockets": len(sockets) or None, "logical_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0)), "smt": smt}
+
+
 def _cpu_baseline(n, p, tau, delta, x, budget_s=20.0):
-    """Multi-core SIMD restatement of the reference engine on this host."""
+    """The CPU restatement of the reference engine timed on this host in the
+    three forms the reference builds: serial (1 core, no SIMD), SIMD on 1
+    core, and SIMD on all cores (standing in for AVX + TBB).  A full
+    evaluation is a likelihood pass plus a gradient pass, as the reference
+    API makes them.  The serial and 1-core forms are timed on the first rows
+    of the same N x N workload (every row holds N pairs, so the rate scales
+    by N / rows); the all-core form on whole evaluations.  Returns the
+    all-core form as the baseline, the others beside it."""
     from oracle import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or oracle.max_threads()
-    # one full evaluation (likelihood pass + gradient pass), repeated while
-    # the budget allows
-    times = []
-    s = g = None
-    while True:
-        t0 = time.perf_counter()
-        s = oracle.simd_sum_of_increments(delta, x, tau, True, threads=threads)
-        g = oracle.simd_gradient(delta, x, tau, True, threads=threads)
-        times.append(time.perf_counter() - t0)
-        if sum(times) + times[-1] > budget_s or len(times) >= 5:
-            break
-    per_eval = float(np.median(times))
     width = oracle.simd_width()
+    simd = f"AVX{'-512' if width == 512 else '2' if width == 256 else ' off'}"
+
+    def timed(fn, repeats, budget):
+        times = []
+        while True:
+            t0 = time.perf_counter()
+            out = fn()
+            times.append(time.perf_counter() - t0)
+            if sum(times) + times[-1] > budget or len(times) >= repeats:
+                return float(np.median(times)), len(times), out
+
+    # all cores, whole evaluations
+    t_all, k_all, (s, g) = timed(lambda: (oracle.simd_sum_of_increments(delta, x, tau, True, threads=threads),
+                                          oracle.simd_gradient(delta, x, tau, True, threads=threads)),
+                                 5, 0.5 * budget_s)
+    # one core, SIMD, a row sample
+    rows1 = max(1, min(n, n // 5))
+    t_one, k_one, _ = timed(lambda: (oracle.simd_sum_of_increments(delta, x, tau, True, threads=1, rows=rows1),
+                                     oracle.simd_gradient(delta, x, tau, True, threads=1, rows=rows1)),
+                            3, 0.2 * budget_s)
+    # one core, serial scalar, a smaller row sample
+    rows0 = max(1, min(n, n // 20))
+    t_ser, k_ser, _ = timed(lambda: (oracle.rows_sum_of_increments(delta, x, tau, True, rows0),
+                                     oracle.rows_gradient(delta, x, tau, True, rows0)),
+                            3, 0.3 * budget_s)
+    variants = {
+        "serial_1core": {"value": rows0 / (n * t_ser), "unit": "evals/s", "cores": 1,
+                         "sample": f"{k_ser} timings of rows 0..{rows0 - 1} (of {n}), median, scaled by "
+                                   f"{n}/{rows0}; scalar, no vectorisation"},
+        "simd_1core": {"value": rows1 / (n * t_one), "unit": "evals/s", "cores": 1,
+                       "sample": f"{k_one} timings of rows 0..{rows1 - 1} (of {n}), median, scaled by "
+                                 f"{n}/{rows1}; {simd}"},
+        "simd_allcores": {"value": 1.0 / t_all, "unit": "evals/s", "cores": threads,
+                          "sample": f"{k_all} full evaluations, median; {simd} + OpenMP, {threads} threads"},
+    }
     return {
-        "value": 1.0 / per_eval,
+        "value": 1.0 / t_all,
         "unit": "evals/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{len(times)} full evaluations (likelihood pass + gradient pass) at N={n}, P={p}, "
-                  f"median; AVX{'-512' if width == 512 else '2' if width == 256 else ' off'} + OpenMP, "
-                  f"{threads} threads",
+        "sample": f"{k_all} full evaluations (likelihood pass + gradient pass) at N={n}, P={p}, "
+                  f"median; {simd} + OpenMP, {threads} threads",
         "sum": s,
         "grad": g,
+        "variants": variants,
     }
 
 
@@ -250,6 +307,48 @@ def _bench_surrogate(iterations=500, n=20000, p=5):
     print(json.dumps(out))
 
 
+def _bench_row_updates(moves=2000, n=10000, p=2):
+    """Single-location moves through the reference API (updateLocations(i, .)
+    then getSumOfIncrements), the O(N) row update, against full sums, at the
+    metric's workload (N = 10000, P = 2, truncation, tau = 2, 5% missing)."""
+    from massivemds_amd import datagen
+    from massivemds_amd.engine import Flags, MultiDimensionalScaling
+
+    delta = datagen.euclid10(n, noise_sd=0.1, missing=0.05)
+    x = datagen.locations(n, p)
+    out = {"workload": f"row_updates_n{n}_p{p}_trunc_tau2_missing5", "moves": moves}
+    for name, fp32 in (("f64", False), ("f32", True)):
+        flags = Flags.OPENCL | Flags.LEFT_TRUNCATION | (Flags.FLOAT if fp32 else 0)
+        e = MultiDimensionalScaling(p, n, flags, 0)
+        e.setPairwiseData(delta)
+        e.setParameters([2.0])
+        xs = x.copy()
+        e.updateLocations(-1, xs)
+        e.getSumOfIncrements()
+        rng = np.random.default_rng(0)
+        rows = rng.integers(0, n, moves)
+        steps = 0.01 * rng.standard_normal((moves, p))
+        t0 = time.perf_counter()
+        for k in range(moves):
+            i = int(rows[k])
+            xs[i] += steps[k]
+            e.updateLocations(i, xs[i])
+            e.getSumOfIncrements()
+        t_row = (time.perf_counter() - t0) / moves
+        s_inc = e.getSumOfIncrements()
+        f, r = e.counters()
+        t0 = time.perf_counter()
+        for _ in range(50):
+            e.makeDirty()
+            s_full = e.getSumOfIncrements()
+        t_full = (time.perf_counter() - t0) / 50
+        e.close()
+        out[name] = {"moves_per_s": 1.0 / t_row, "us_per_move": t_row * 1e6, "full_sum_us": t_full * 1e6,
+                     "speedup_vs_full": t_full / t_row, "row_updates": r, "full_evaluations": f,
+                     "rel_err_vs_full_recompute": abs(s_inc - s_full) / abs(s_full)}
+    print(json.dumps(out))
+
+
 def _calibrate_copy(mib):
     """A plain elementwise copy kernel of known size (torch, as plumbing)."""
     import torch
@@ -271,9 +370,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--workload", default="n10000", choices=["n10000", "batch64", "hmc5000", "surrogate20000"],
+    ap.add_argument("--workload", default="n10000",
+                    choices=["n10000", "batch64", "hmc5000", "surrogate20000", "rowupdate10000"],
                     help="n10000: the metric's workload (default); batch64: the uneven batch; "
-                         "hmc5000: plain HMC; surrogate20000: surrogate against plain HMC")
+                         "hmc5000: plain HMC; surrogate20000: surrogate against plain HMC; "
+                         "rowupdate10000: single-location moves through the O(N) update")
     ap.add_argument("--iterations", type=int, default=500, help="sampling iterations (surrogate20000)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch64)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -291,6 +392,9 @@ def main():
         return
     if args.workload == "surrogate20000":
         _bench_surrogate(args.iterations)
+        return
+    if args.workload == "rowupdate10000":
+        _bench_row_updates()
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a "
@@ -350,8 +454,12 @@ def main():
         rel_s = abs(cpu["sum"] - r64["sum"]) / abs(cpu["sum"])
         rel_g = float(np.max(np.abs(cpu["grad"] - r64["grad"])) / np.max(np.abs(cpu["grad"])))
         out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_variants"] = cpu["variants"]
+        out["cpu_host"] = _cpu_host()
         out["gpu_vs_cpu"] = {"speedup": out["value"] / cpu["value"], "rel_err_sum": rel_s,
-                             "rel_err_grad": rel_g}
+                             "rel_err_grad": rel_g,
+                             "speedup_vs_serial_1core": out["value"] / cpu["variants"]["serial_1core"]["value"],
+                             "speedup_vs_simd_1core": out["value"] / cpu["variants"]["simd_1core"]["value"]}
     print(json.dumps(out))
 
 

@@ -119,6 +119,11 @@ int mds_hip_synchronize(mds_hip_handle h);
  * get_timing synchronises and returns their sums (ms) and the count. */
 int mds_hip_set_timing(mds_hip_handle h, int enable);
 int mds_hip_get_timing(mds_hip_handle h, double* tileMs, double* evalMs, long long* count);
+/* Work counters since creation: full O(N^2) evaluations launched, and sums
+ * obtained by the O(N) single-row update (updateLocations(i, ...) of a state
+ * whose sum was known; no counterpart in the reference, whose incremental
+ * path is disabled, src/NewMultiDimensionalScaling.hpp:563-566). */
+int mds_hip_get_counters(mds_hip_handle h, long long* fullEvaluations, long long* rowUpdates);
 
 /* Layout facts: 1 when the stored observations are symmetric (one tile
  * plane), bytes of packed observations on the device, number of tiles. */

@@ -138,6 +138,13 @@ int mds_hip_get_timing(mds_hip_handle h, double* tileMs, double* evalMs, long lo
     return guarded(h, [&](mdship::Engine& e) { e.getTiming(tileMs, evalMs, count); });
 }
 
+int mds_hip_get_counters(mds_hip_handle h, long long* fullEvaluations, long long* rowUpdates) {
+    return guarded(h, [&](mdship::Engine& e) {
+        if (fullEvaluations) *fullEvaluations = e.fullEvaluations;
+        if (rowUpdates) *rowUpdates = e.rowUpdates;
+    });
+}
+
 int mds_hip_get_layout(mds_hip_handle h, int* symmetric, long long* observationBytes,
                        long long* tiles) {
     return guarded(h, [&](mdship::Engine& e) {

@@ -10,8 +10,14 @@
 //     restoreState restores sum and precision, marks the sum known and swaps
 //     the location buffers (:234-255);
 //   * setPairwiseData does not touch the cached sum (:257-260);
-//   * every evaluation is a full O(N^2) pass (the reference's incremental
-//     path is dead code, :563-566).
+//   * a single-location update (updateLocations(i, ...)) of a state whose
+//     sum is known is followed by an O(N) update of the sum
+//     (mds_row.hip), the path the reference sketches and disables
+//     (:146-157, :563-616); repeated updates of the same row chain from the
+//     same base, a second row forces a full recomputation (:150-153), and so
+//     do data changes, makeDirty, and every kMaxRowUpdates-th consecutive
+//     incremental update (bounding the accumulated rounding);
+//   * every other evaluation is a full O(N^2) pass.
 // In addition the engine keeps the device gradient of the last fused
 // evaluation and reuses it while the state is unchanged (the reference
 // recomputes it; the values are identical), and a gradient call fills the
@@ -62,6 +68,24 @@ struct Engine {
     bool devSumFresh = false;  // dSum matches the current state
     bool gradKnown = false;    // dGrad matches the current state
 
+    // single-row updates: the known sum of the state before row pendingRow
+    // moved (its old location in dOldRow) is baseSum
+    static constexpr int kMaxRowUpdates = 1024;
+    bool sumFromCurrentData = false;  // the known sum was computed on the current observations
+    int pendingRow = -1;
+    double baseSum = 0.0;
+    int rowUpdatesInARow = 0;
+    void* dOldRow = nullptr;
+    double* dRowPartial = nullptr;
+    double* hRowPartial = nullptr;  // pinned
+    // single-row uploads: two pinned staging slots, each reused only after
+    // its previous copy has completed (so the call need not wait for it)
+    void* hRowStage = nullptr;
+    hipEvent_t rowStageDone[2] = {nullptr, nullptr};
+    bool rowStageBusy[2] = {false, false};
+    int rowStageSlot = 0;
+    long long fullEvaluations = 0, rowUpdates = 0;
+
     bool timing = false;
     std::vector<hipEvent_t> eventPool;
     std::vector<hipEvent_t> pending;  // triples: start, after tiles, after reduce
@@ -110,6 +134,13 @@ struct Engine {
             check(hipHostMalloc((void**)&hSum, sizeof(double), hipHostMallocDefault), "hipHostMalloc(sum)");
             check(hipHostMalloc(&hX, (size_t)npad * kdim * realSize, hipHostMallocDefault),
                   "hipHostMalloc(locations)");
+            check(hipMalloc(&dOldRow, kdim * realSize), "hipMalloc(old row)");
+            check(hipMalloc((void**)&dRowPartial, row_delta_blocks(N) * sizeof(double)), "hipMalloc(row sums)");
+            check(hipHostMalloc((void**)&hRowPartial, row_delta_blocks(N) * sizeof(double), hipHostMallocDefault),
+                  "hipHostMalloc(row sums)");
+            check(hipHostMalloc(&hRowStage, 2 * kdim * realSize, hipHostMallocDefault), "hipHostMalloc(row)");
+            for (int k = 0; k < 2; ++k)
+                check(hipEventCreateWithFlags(&rowStageDone[k], hipEventDisableTiming), "hipEventCreate");
             std::memset(hX, 0, (size_t)npad * kdim * realSize);
             check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         } catch (...) {
@@ -138,7 +169,17 @@ struct Engine {
         (void)hipHostFree(hGrad);
         (void)hipHostFree(hSum);
         (void)hipHostFree(hX);
+        (void)hipFree(dOldRow);
+        (void)hipFree(dRowPartial);
+        (void)hipHostFree(hRowPartial);
+        (void)hipHostFree(hRowStage);
+        for (int k = 0; k < 2; ++k)
+            if (rowStageDone[k]) (void)hipEventDestroy(rowStageDone[k]);
+        rowStageDone[0] = rowStageDone[1] = nullptr;
+        hRowStage = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
+        dOldRow = nullptr;
+        dRowPartial = hRowPartial = nullptr;
         dDelta = dX[0] = dX[1] = nullptr;
         dTiles = nullptr;
         dTileSums = dPart = dGrad = dSum = hGrad = hSum = nullptr;
@@ -168,22 +209,81 @@ struct Engine {
             rows = 1;
             row0 = index;
         }
-        // Stage as [rows][kdim] in the engine's real type, padding with zeros.
+        // Stage as [rows][kdim] in the engine's real type, padding with zeros:
+        // a single row in the next free row slot, all rows in hX.
+        const int slot = rowStageSlot;
+        void* stage = hX;
+        if (rows == 1) {
+            if (rowStageBusy[slot]) check(hipEventSynchronize(rowStageDone[slot]), "hipEventSynchronize");
+            stage = (char*)hRowStage + slot * kdim * realSize;
+        }
         if (fp32) {
-            float* s = (float*)hX;
+            float* s = (float*)stage;
             for (size_t r = 0; r < rows; ++r)
                 for (int k = 0; k < kdim; ++k) s[r * kdim + k] = k < P ? (float)loc[r * P + k] : 0.0f;
         } else {
-            double* s = (double*)hX;
+            double* s = (double*)stage;
             for (size_t r = 0; r < rows; ++r)
                 for (int k = 0; k < kdim; ++k) s[r * kdim + k] = k < P ? loc[r * P + k] : 0.0;
         }
         char* dst = (char*)dX[cur] + row0 * kdim * realSize;
-        check(hipMemcpyAsync(dst, hX, rows * kdim * realSize, hipMemcpyHostToDevice, stream),
+        if (index >= 0 && sumKnown && sumFromCurrentData) {
+            // start a single-row update from the known sum: keep the old row
+            check(hipMemcpyAsync(dOldRow, dst, kdim * realSize, hipMemcpyDeviceToDevice, stream),
+                  "hipMemcpy(old row)");
+            pendingRow = index;
+            baseSum = sumOfIncrements;
+        } else if (index < 0 || index != pendingRow) {
+            pendingRow = -1;  // several rows (or all) moved: full recomputation (:150-153)
+        }
+        check(hipMemcpyAsync(dst, stage, rows * kdim * realSize, hipMemcpyHostToDevice, stream),
               "hipMemcpy(locations)");
-        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (rows == 1) {
+            // the caller's row is already copied; the slot is reused once this copy is done
+            check(hipEventRecord(rowStageDone[slot], stream), "hipEventRecord");
+            rowStageBusy[slot] = true;
+            rowStageSlot = 1 - slot;
+        } else {
+            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        }
         sumKnown = false;
         invalidate_device_results();
+    }
+
+    // The raw sum after the pending single-row update: base + half the
+    // row's change, the block partials added in order.
+    void applyRowUpdate() {
+        RowDeltaLaunch a;
+        a.delta = dDelta;
+        a.x = dX[cur];
+        a.xOld = dOldRow;
+        a.row = pendingRow;
+        a.n = N;
+        a.nb = nb;
+        a.kdim = kdim;
+        a.planeStride = (int64_t)numTiles * kTileElems;
+        a.tau = precision;
+        a.truncated = truncated;
+        a.symmetric = symmetric;
+        a.fp32 = fp32;
+        a.partial = dRowPartial;
+        check(launch_row_delta(a, stream), "row update kernel");
+        const int blocks = row_delta_blocks(N);
+        check(hipMemcpyAsync(hRowPartial, dRowPartial, blocks * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "hipMemcpy(row sums)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        double change = 0.0;
+        for (int b = 0; b < blocks; ++b) change += hRowPartial[b];
+        sumOfIncrements = baseSum + 0.5 * change;
+        ++rowUpdates;
+        ++rowUpdatesInARow;
+    }
+
+    // A full evaluation's sum became the known sum.
+    void sumFromFullEvaluation() {
+        sumFromCurrentData = true;
+        pendingRow = -1;
+        rowUpdatesInARow = 0;
     }
 
     void setPairwiseData(const double* data, size_t length) {
@@ -212,6 +312,8 @@ struct Engine {
         check(hipMemcpy(dDelta, host.get(), bytes, hipMemcpyHostToDevice), "hipMemcpy(observations)");
         symmetric = sym;
         dataSet = true;
+        sumFromCurrentData = false;  // the cached sum stays (:257-260) but is no base for row updates
+        pendingRow = -1;
         invalidate_device_results();
     }
 
@@ -223,12 +325,16 @@ struct Engine {
             throw Error(MDS_HIP_ERR_ARGUMENT, "setParameters: precision must be finite and > 0");
         precision = data[0];
         precisionSet = true;
-        if (truncated) sumKnown = false;
+        if (truncated) {
+            sumKnown = false;
+            pendingRow = -1;  // the base sum was for another precision
+        }
         invalidate_device_results();
     }
 
     void makeDirty() {  // :276-279
         sumKnown = false;
+        pendingRow = -1;
         invalidate_device_results();
     }
 
@@ -243,6 +349,7 @@ struct Engine {
     void restoreState() {  // :234-255
         sumOfIncrements = storedSumOfIncrements;
         sumKnown = true;
+        pendingRow = -1;
         precision = storedPrecision;
         cur = 1 - cur;
         invalidate_device_results();
@@ -309,15 +416,22 @@ struct Engine {
         }
         devSumFresh = true;
         if (gradient) gradKnown = true;
+        ++fullEvaluations;
     }
 
     double getSumOfIncrements() {  // :193-203
         if (!sumKnown) {
-            if (!devSumFresh) evaluate(false);
-            check(hipMemcpyAsync(hSum, dSum, sizeof(double), hipMemcpyDeviceToHost, stream),
-                  "hipMemcpy(sum)");
-            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-            sumOfIncrements = *hSum;
+            if (pendingRow >= 0 && !devSumFresh && rowUpdatesInARow < kMaxRowUpdates) {
+                applyRowUpdate();
+                pendingRow = -1;
+            } else {
+                if (!devSumFresh) evaluate(false);
+                check(hipMemcpyAsync(hSum, dSum, sizeof(double), hipMemcpyDeviceToHost, stream),
+                      "hipMemcpy(sum)");
+                check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+                sumOfIncrements = *hSum;
+                sumFromFullEvaluation();
+            }
             sumKnown = true;
         }
         return truncated ? sumOfIncrements : 0.5 * precision * sumOfIncrements;
@@ -339,6 +453,7 @@ struct Engine {
         if (fillSum) {
             sumOfIncrements = *hSum;
             sumKnown = true;
+            sumFromFullEvaluation();
         }
     }
 

@@ -55,6 +55,32 @@ struct ReduceLaunch {
     double* sumOut;          // 1 double: 0.5 * sum of tile sums
 };
 
+// Change of the raw sum of increments when one location row moves (O(N)):
+// per block of kRowThreads locations j, the partial
+// sum_j [term(delta_ij, d_new) - term(delta_ij, d_old) + the same for delta_ji]
+// over observed entries, written to partial[block] (the caller adds the
+// partials in order and halves).
+constexpr int kRowThreads = 256;
+
+struct RowDeltaLaunch {
+    const void* delta;       // packed tiles as in TileLaunch
+    const void* x;           // Real [nb * 64][kdim]: row `row` already holds the new location
+    const void* xOld;        // Real [kdim]: the row's previous location
+    int row;
+    int n;
+    int nb;
+    int kdim;
+    int64_t planeStride;
+    double tau;
+    bool truncated;
+    bool symmetric;
+    bool fp32;               // Real = float (else double)
+    double* partial;         // [blocks(n)]
+};
+
+inline int row_delta_blocks(int n) { return (n + kRowThreads - 1) / kRowThreads; }
+
+hipError_t launch_row_delta(const RowDeltaLaunch& a, hipStream_t s);
 hipError_t launch_tiles_f64(const TileLaunch& a, hipStream_t s);
 hipError_t launch_tiles_f32(const TileLaunch& a, hipStream_t s);
 hipError_t launch_reduce(const ReduceLaunch& a, hipStream_t s);

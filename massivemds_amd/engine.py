@@ -80,6 +80,8 @@ def load_library() -> ctypes.CDLL:
                                ctypes.c_int),
         "mds_hip_get_layout": ([H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong),
                                 ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+        "mds_hip_get_counters": ([H, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
+                                 ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -198,6 +200,12 @@ class MultiDimensionalScaling:
         self._check(self._lib.mds_hip_get_timing(self._h, ctypes.byref(a), ctypes.byref(b),
                                                  ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def counters(self):
+        """(full O(N^2) evaluations launched, sums from O(N) single-row updates)."""
+        f, r = ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self._lib.mds_hip_get_counters(self._h, ctypes.byref(f), ctypes.byref(r)))
+        return f.value, r.value
 
     def layout(self):
         """(symmetric, packed observation bytes, tile count)."""

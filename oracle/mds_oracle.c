@@ -61,6 +61,11 @@ static float distance_f(const float* xi, const float* xj, int p) {
     return sqrtf(sum);
 }
 
+double mds_oracle_rows_sum_of_increments_f64(const double* obs, const double* x, int64_t n, int p,
+                                             double precision, int truncated, int64_t rows);
+void mds_oracle_rows_gradient_f64(const double* obs, const double* x, int64_t n, int p,
+                                  double precision, int truncated, int64_t rows, double* gradient);
+
 /*
  * FP64 getSumOfIncrements() on a freshly computed state.
  * src/NewMultiDimensionalScaling.hpp:524-561: row sums over j (left to
@@ -70,10 +75,20 @@ static float distance_f(const float* xi, const float* xj, int p) {
 double mds_oracle_sum_of_increments_f64(const double* obs, const double* x,
                                         int64_t n, int p, double precision,
                                         int truncated) {
+    return mds_oracle_rows_sum_of_increments_f64(obs, x, n, p, precision, truncated, n);
+}
+
+/* The same over the first `rows` rows only: a bounded sample of the serial
+ * engine's work, for timing.  As in mds_oracle_simd_sum_of_increments, a
+ * sample (rows < n) returns the partial, unhalved and unscaled, sum. */
+double mds_oracle_rows_sum_of_increments_f64(const double* obs, const double* x,
+                                             int64_t n, int p, double precision,
+                                             int truncated, int64_t rows) {
     const double scale = 0.5 * precision;
     const double oneOverSd = sqrt(precision);
+    const int64_t m = (rows > 0 && rows < n) ? rows : n;
     double total = 0.0;
-    for (int64_t i = 0; i < n; ++i) {
+    for (int64_t i = 0; i < m; ++i) {
         double rowSum = 0.0;
         for (int64_t j = 0; j < n; ++j) {
             const double observation = obs[i * n + j];
@@ -89,6 +104,7 @@ double mds_oracle_sum_of_increments_f64(const double* obs, const double* x,
         }
         total += rowSum;
     }
+    if (m < n) return total;
     total /= 2.0;
     return truncated ? total : 0.5 * precision * total;
 }
@@ -96,10 +112,18 @@ double mds_oracle_sum_of_increments_f64(const double* obs, const double* x,
 /* src/NewMultiDimensionalScaling.hpp:304-334 with :421-461 (double). */
 void mds_oracle_gradient_f64(const double* obs, const double* x, int64_t n, int p,
                              double precision, int truncated, double* gradient) {
+    mds_oracle_rows_gradient_f64(obs, x, n, p, precision, truncated, n, gradient);
+}
+
+/* Gradient rows [0, rows) only (rows <= 0: all), for timing. */
+void mds_oracle_rows_gradient_f64(const double* obs, const double* x, int64_t n, int p,
+                                  double precision, int truncated, int64_t rows,
+                                  double* gradient) {
     const double scale = precision;
     const double sqrtScale = sqrt(scale);
+    const int64_t m = (rows > 0 && rows < n) ? rows : n;
     memset(gradient, 0, sizeof(double) * (size_t)(n * p));
-    for (int64_t i = 0; i < n; ++i) {
+    for (int64_t i = 0; i < m; ++i) {
         for (int64_t j = 0; j < n; ++j) {
             const double observation = obs[i * n + j];
             if (i == j || isnan(observation)) continue;

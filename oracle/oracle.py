@@ -50,6 +50,12 @@ def load() -> ctypes.CDLL:
     lib.mds_oracle_gradient_f64.restype = None
     lib.mds_oracle_gradient_f32.argtypes = lib.mds_oracle_gradient_f64.argtypes
     lib.mds_oracle_gradient_f32.restype = None
+    lib.mds_oracle_rows_sum_of_increments_f64.argtypes = [_dp, _dp, i64, ctypes.c_int, ctypes.c_double,
+                                                          ctypes.c_int, i64]
+    lib.mds_oracle_rows_sum_of_increments_f64.restype = ctypes.c_double
+    lib.mds_oracle_rows_gradient_f64.argtypes = [_dp, _dp, i64, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                 i64, _dp]
+    lib.mds_oracle_rows_gradient_f64.restype = None
     lib.mds_oracle_log_likelihood.argtypes = [ctypes.c_double, i64, ctypes.c_double]
     lib.mds_oracle_log_likelihood.restype = ctypes.c_double
     lib.mds_oracle_simd_sum_of_increments.argtypes = [_dp, _dp, i64, ctypes.c_int, ctypes.c_double,
@@ -88,6 +94,31 @@ def gradient(delta, x, precision, truncated, fp32=False) -> np.ndarray:
     out = np.empty(n * p)
     fn = lib.mds_oracle_gradient_f32 if fp32 else lib.mds_oracle_gradient_f64
     fn(pd, px, n, p, float(precision), int(bool(truncated)), out.ctypes.data_as(_dp))
+    return out.reshape(n, p)
+
+
+def rows_sum_of_increments(delta, x, precision, truncated, rows) -> float:
+    """Serial FP64 sum over the first `rows` rows (timing samples): with
+    rows < N the raw row-sum total (unhalved, unscaled), as
+    simd_sum_of_increments returns for a sample."""
+    lib = load()
+    delta, pd = _c(delta)
+    x, px = _c(x)
+    n = x.shape[0]
+    return lib.mds_oracle_rows_sum_of_increments_f64(pd, px, n, x.size // n, float(precision),
+                                                     int(bool(truncated)), int(rows))
+
+
+def rows_gradient(delta, x, precision, truncated, rows) -> np.ndarray:
+    """Serial FP64 gradient rows [0, rows) (timing samples; other rows 0)."""
+    lib = load()
+    delta, pd = _c(delta)
+    x, px = _c(x)
+    n = x.shape[0]
+    p = x.size // n
+    out = np.empty(n * p)
+    lib.mds_oracle_rows_gradient_f64(pd, px, n, p, float(precision), int(bool(truncated)), int(rows),
+                                     out.ctypes.data_as(_dp))
     return out.reshape(n, p)
 
 

@@ -1,0 +1,132 @@
+"""GPU tests of the O(N) single-location update of the sum of increments
+(mds_row.hip, Engine::applyRowUpdate).  The reference disables this path
+(src/NewMultiDimensionalScaling.hpp:563-566) and recomputes; the engine must
+give the recomputed value (FP64: 1e-12 relative; FP32: its own full
+evaluation's accuracy) and keep the reference's state rules: a second row
+forces a full recomputation (:150-153), store/restore behave as for full
+updates (:205-255)."""
+import numpy as np
+import pytest
+
+from massivemds_amd import datagen
+from massivemds_amd.engine import Flags, MultiDimensionalScaling
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(n, p, truncated, fp32, delta, x, tau=1.7):
+    flags = Flags.OPENCL | (Flags.LEFT_TRUNCATION if truncated else 0) | (Flags.FLOAT if fp32 else 0)
+    e = MultiDimensionalScaling(p, n, flags, 0)
+    e.setPairwiseData(delta)
+    e.setParameters([tau])
+    e.updateLocations(-1, x)
+    return e
+
+
+def _fresh_sum(n, p, truncated, fp32, delta, x, tau=1.7):
+    e = _engine(n, p, truncated, fp32, delta, x, tau)
+    s = e.getSumOfIncrements()
+    e.close()
+    return s
+
+
+@pytest.mark.parametrize("n,p,truncated,asym,fp32", [
+    (300, 2, True, False, False), (300, 2, False, False, False), (257, 3, True, True, False),
+    (130, 5, True, False, False), (1000, 2, True, True, False), (300, 2, True, False, True)])
+def test_single_row_updates_match_recomputation(n, p, truncated, asym, fp32):
+    print(f"row updates n={n} p={p} T={truncated} asym={asym} fp32={fp32}", flush=True)
+    delta = datagen.euclid10(n, 21, 0.1, 0.05)
+    if asym:
+        delta = delta + 0.05 * np.triu(np.abs(datagen.normals(21, 9, n * n).reshape(n, n)), 1)
+        delta[3, 7] = np.nan  # observed in one direction only
+    x = datagen.locations(n, p, 22)
+    e = _engine(n, p, truncated, fp32, delta, x)
+    e.getSumOfIncrements()
+    full0, rows0 = e.counters()
+    rng = np.random.default_rng(n + p)
+    tol = 2e-6 if fp32 else 1e-12
+    for step in range(24):
+        i = int(rng.integers(0, n))
+        x[i] = x[i] + 0.3 * rng.standard_normal(p)
+        e.updateLocations(i, x[i])
+        if step % 3 == 2:  # the same row again: chains from the same base
+            x[i] = x[i] + 0.1 * rng.standard_normal(p)
+            e.updateLocations(i, x[i])
+        s = e.getSumOfIncrements()
+        ref = _fresh_sum(n, p, truncated, fp32, delta, x)
+        assert abs(s - ref) <= tol * abs(ref), (step, s, ref)
+    full1, rows1 = e.counters()
+    assert rows1 - rows0 == 24 and full1 == full0  # every sum came from the O(N) path
+    e.close()
+
+
+def test_state_rules():
+    n, p = 200, 2
+    delta = datagen.euclid10(n, 31, 0.1, 0.02)
+    x = datagen.locations(n, p, 32)
+    e = _engine(n, p, True, False, delta, x)
+    s0 = e.getSumOfIncrements()
+    f, r = e.counters()
+    # two different rows: full recomputation (:150-153)
+    x[4] += 0.2
+    x[9] -= 0.2
+    e.updateLocations(4, x[4])
+    e.updateLocations(9, x[9])
+    s1 = e.getSumOfIncrements()
+    assert e.counters() == (f + 1, r)
+    assert abs(s1 - _fresh_sum(n, p, True, False, delta, x)) <= 1e-12 * abs(s1)
+    # store, move one row, restore: the stored sum and locations come back
+    e.storeState()
+    x_old = x[17].copy()
+    x[17] += 0.5
+    e.updateLocations(17, x[17])
+    s2 = e.getSumOfIncrements()
+    assert e.counters() == (f + 1, r + 1) and s2 != s1
+    e.restoreState()
+    assert e.getSumOfIncrements() == s1
+    x[17] = x_old
+    # after restore the sum is known again: the next move is O(N)
+    x[3] += 0.25
+    e.updateLocations(3, x[3])
+    s3 = e.getSumOfIncrements()
+    assert e.counters() == (f + 1, r + 2)
+    assert abs(s3 - _fresh_sum(n, p, True, False, delta, x)) <= 1e-12 * abs(s3)
+    # new data: the cached sum stays (:257-260) but is no base for the O(N) path
+    e.setPairwiseData(delta * 1.01)
+    x[5] += 0.1
+    e.updateLocations(5, x[5])
+    s4 = e.getSumOfIncrements()
+    assert e.counters() == (f + 2, r + 2)
+    assert abs(s4 - _fresh_sum(n, p, True, False, delta * 1.01, x)) <= 1e-12 * abs(s4)
+    # a gradient call on a moved row computes everything (and the sum with it)
+    x[6] += 0.1
+    e.updateLocations(6, x[6])
+    g = e.getLogLikelihoodGradient()
+    assert e.counters() == (f + 3, r + 2)
+    assert np.all(np.isfinite(g))
+    assert s0 != s1
+    e.close()
+
+
+def test_many_updates_stay_exact_and_refresh():
+    """A long chain of accepted single-row moves (more than the limit of
+    consecutive O(N) updates) stays within 1e-12 of a recomputation."""
+    n, p = 400, 2
+    delta = datagen.euclid10(n, 41, 0.1, 0.05)
+    x = datagen.locations(n, p, 42)
+    e = _engine(n, p, True, False, delta, x)
+    e.getSumOfIncrements()
+    f0, r0 = e.counters()
+    rng = np.random.default_rng(5)
+    moves = 1100
+    for _ in range(moves):
+        i = int(rng.integers(0, n))
+        x[i] = x[i] + 0.05 * rng.standard_normal(p)
+        e.updateLocations(i, x[i])
+        e.getSumOfIncrements()
+    f1, r1 = e.counters()
+    assert f1 - f0 == 1 and r1 - r0 == moves - 1  # one refresh after 1024 O(N) updates
+    s = e.getSumOfIncrements()
+    ref = _fresh_sum(n, p, True, False, delta, x)
+    assert abs(s - ref) <= 1e-12 * abs(ref)
+    e.close()

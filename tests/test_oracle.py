@@ -219,3 +219,21 @@ def test_simd_baseline_matches_serial(oracle, n, p, truncated):
     # a row-limited evaluation returns the partial sums of the first rows
     part = oracle.simd_gradient(delta, x, 1.5, truncated, threads=2, rows=10)
     np.testing.assert_allclose(part[:10], g[:10], rtol=1e-12, atol=1e-12 * np.max(np.abs(g)))
+
+
+def test_row_samples_match_full_and_simd(oracle):
+    """The timing samples of the CPU baselines: the serial oracle over the
+    first rows gives the full result's rows (gradient) and the same raw
+    partial sum as the SIMD oracle's sample; all rows is the full result."""
+    n, p = 150, 3
+    delta = datagen.euclid10(n, 9, 0.1, 0.05)
+    x = datagen.locations(n, p)
+    for truncated in (True, False):
+        full = oracle.sum_of_increments(delta, x, 1.7, truncated)
+        assert oracle.rows_sum_of_increments(delta, x, 1.7, truncated, n) == full
+        part = oracle.rows_sum_of_increments(delta, x, 1.7, truncated, 40)
+        simd = oracle.simd_sum_of_increments(delta, x, 1.7, truncated, threads=1, rows=40)
+        assert abs(part - simd) <= 1e-13 * abs(part)
+        g = oracle.gradient(delta, x, 1.7, truncated)
+        gr = oracle.rows_gradient(delta, x, 1.7, truncated, 40)
+        assert np.array_equal(gr[:40], g[:40]) and np.all(gr[40:] == 0.0)
