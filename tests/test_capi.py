@@ -27,10 +27,11 @@ def _lib():
     return ctypes.CDLL(str(eng.library_path()))
 
 
-def test_header_symbols_exported():
+@pytest.mark.parametrize("header,minimum", [("mds_hip.h", 20), ("mds_hip_singleton.h", 14)])
+def test_header_symbols_exported(header, minimum):
     lib = _lib()
-    names = _declared("mds_hip.h")
-    assert len(names) >= 20
+    names = _declared(header)
+    assert len(names) >= minimum
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
@@ -82,3 +83,27 @@ def test_cpu_routes_are_not_served():
         eng.factory(2, 10, eng.Flags.TBB, 0, 4)
     with pytest.raises(ValueError):
         mds.createEngine(2, 10, truncation=True, gpu=0)
+
+
+def test_singleton_registry_errors_without_gpu():
+    """The BEAST bridge's registry refuses bad instance numbers and CPU flag
+    sets with a status and a message (no GPU needed)."""
+    lib = _lib()
+    lib.mds_hip_singleton_last_error.restype = ctypes.c_char_p
+    lib.mds_hip_singleton_get_sum_of_increments.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    s = ctypes.c_double()
+    assert lib.mds_hip_singleton_get_sum_of_increments(12345, ctypes.byref(s)) == 1
+    assert b"no instance" in lib.mds_hip_singleton_last_error(-1)
+    inst = ctypes.c_int(-7)
+    lib.mds_hip_singleton_initialize.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_long, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    assert lib.mds_hip_singleton_initialize(2, 10, 1 << 5, 0, 1, ctypes.byref(inst)) == 1  # no GPU bit
+    assert b"GPU engine only" in lib.mds_hip_singleton_last_error(-1)
+    assert inst.value == -7
+
+
+def test_singleton_harness_builds(tmp_path):
+    from tests.singleton_harness import harness
+
+    exe = harness(tmp_path)
+    assert exe.exists()
