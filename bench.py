@@ -349,6 +349,44 @@ def _bench_row_updates(moves=2000, n=10000, p=2):
     print(json.dumps(out))
 
 
+def _bench_beast(steps=2000, warmup=20):
+    """Real data: the 372 H3 influenza antigenic distances and the BEAST tip
+    locations of two logged states (the reference's inst/extdata, committed
+    as tests/golden/beast_h3_kat.npz by make_beast_kat.py, read with
+    massivemds_amd.beastio), non-truncated as BEAST ran it, at the logged
+    precision: the log-likelihood against BEAST's logged value, and fused
+    likelihood + gradient evaluations per second (the whole problem is
+    1.1 MB: launch-bound, not a roofline case)."""
+    from pathlib import Path
+
+    from massivemds_amd import mds
+
+    z = np.load(Path(__file__).resolve().parent / "tests" / "golden" / "beast_h3_kat.npz")
+    delta = z["delta_x1e4"].astype(np.float64) / 10000.0
+    out = {"workload": "beast_h3_n372_p2_notrunc", "N": int(delta.shape[0]), "P": 2, "steps": steps}
+    for tag, xk, tk, lk in (("state1", "locations", "precision", "loglik"),
+                            ("state2", "locations2", "precision2", "loglik2")):
+        eng = mds.createEngine(2, delta.shape[0], truncation=False, gpu=1)
+        mds.setPairwiseData(eng, delta)
+        mds.updateLocations(eng, z[xk])
+        mds.setPrecision(eng, float(z[tk]))
+        ll = mds.getLogLikelihood(eng)
+        e = eng.engine
+        for _ in range(warmup):
+            e.evaluate_async(True)
+        e.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            e.evaluate_async(True)
+        e.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[tag] = {"loglik": ll, "beast_loglik": float(z[lk]), "abs_err": abs(ll - float(z[lk])),
+                    "rel_err": abs(ll - float(z[lk])) / abs(float(z[lk])), "evals_per_s": 1.0 / dt,
+                    "us_per_eval": dt * 1e6}
+        e.close()
+    print(json.dumps(out))
+
+
 def _calibrate_copy(mib):
     """A plain elementwise copy kernel of known size (torch, as plumbing)."""
     import torch
@@ -371,10 +409,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--workload", default="n10000",
-                    choices=["n10000", "batch64", "hmc5000", "surrogate20000", "rowupdate10000"],
+                    choices=["n10000", "batch64", "hmc5000", "surrogate20000", "rowupdate10000", "beast372"],
                     help="n10000: the metric's workload (default); batch64: the uneven batch; "
                          "hmc5000: plain HMC; surrogate20000: surrogate against plain HMC; "
-                         "rowupdate10000: single-location moves through the O(N) update")
+                         "rowupdate10000: single-location moves through the O(N) update; "
+                         "beast372: real BEAST data (N = 372) against the logged likelihood")
     ap.add_argument("--iterations", type=int, default=500, help="sampling iterations (surrogate20000)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch64)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -395,6 +434,9 @@ def main():
         return
     if args.workload == "rowupdate10000":
         _bench_row_updates()
+        return
+    if args.workload == "beast372":
+        _bench_beast()
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a "
