@@ -85,30 +85,42 @@ struct LaneSum;
 
 // add(v) adds a value; add_product(a, b) adds a * b with one rounding (an
 // explicit fma, so that the result never depends on whether the compiler
-// fuses a product into the sum).
+// fuses a product into the sum).  flush() closes a group of steps; value()
+// is read after a flush.
+//
+// FP64: a plain sum.  FP32: blocked compensated summation: within a group the
+// terms go into a plain FP32 group sum g, and each flush adds g into a
+// Kahan-compensated (s, c) pair, so the compensation costs four operations
+// per group instead of per term.
 template <>
 struct LaneSum<double> {
     double s = 0.0;
     __device__ __forceinline__ void add(double v) { s += v; }
     __device__ __forceinline__ void add_product(double a, double b) { s = fma(a, b, s); }
+    __device__ __forceinline__ void flush() {}
     __device__ __forceinline__ double value() const { return s; }
 };
+
+// (s, c) += g with the Kahan step (c holds the negated rounding error)
+__device__ __forceinline__ void kahan_add(float& s, float& c, float g) {
+    const float y = g - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
 
 template <>
 struct LaneSum<float> {
     float s = 0.0f;
     float c = 0.0f;
-    __device__ __forceinline__ void add(float v) { add_compensated(v - c); }
-    __device__ __forceinline__ void add_product(float a, float b) { add_compensated(fmaf(a, b, -c)); }
-    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
-
-  private:
-    // y = v - c, the value corrected by the running compensation
-    __device__ __forceinline__ void add_compensated(float y) {
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
+    float g = 0.0f;
+    __device__ __forceinline__ void add(float v) { g += v; }
+    __device__ __forceinline__ void add_product(float a, float b) { g = fmaf(a, b, g); }
+    __device__ __forceinline__ void flush() {
+        kahan_add(s, c, g);
+        g = 0.0f;
     }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
 __device__ __forceinline__ bool m_isnan(double v) { return v != v; }
@@ -118,7 +130,9 @@ __device__ __forceinline__ bool m_isnan(float v) { return v != v; }
 // the partial sum of tile column (l + k) mod 64; rotate() moves every
 // accumulator one lane down (DPP wave_rol:1, lane l <- lane l + 1) so that
 // at step k + 1 lane l holds column (l + k + 1) mod 64.  After 64 steps lane
-// l holds column l.
+// l holds column l.  FP32 rotates only the group sum g each step; the
+// compensated pair stays put during a group of STEPS steps and moves STEPS
+// lanes at the flush (one ds_bpermute each), landing on g's columns.
 constexpr int kDppWaveRol1 = 0x134;
 
 template <typename Real>
@@ -130,19 +144,34 @@ struct ColSum<double> {
     __device__ __forceinline__ void add(double v) { s += v; }
     __device__ __forceinline__ void add_product(double a, double b) { s = fma(a, b, s); }
     __device__ __forceinline__ void rotate() { s = dpp_rotate<kDppWaveRol1>(s); }
+    template <int STEPS>
+    __device__ __forceinline__ void flush() {}
     __device__ __forceinline__ double value() const { return s; }
 };
 
+// lane l receives v from lane (l + SHIFT) mod 64 (LDS crossbar, no memory)
+template <int SHIFT>
+__device__ __forceinline__ float lane_rotate(float v) {
+    const int src = ((int)(threadIdx.x % kWave) + SHIFT) & (kWave - 1);
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+
 template <>
 struct ColSum<float> {
-    LaneSum<float> k;
-    __device__ __forceinline__ void add(float v) { k.add(v); }
-    __device__ __forceinline__ void add_product(float a, float b) { k.add_product(a, b); }
-    __device__ __forceinline__ void rotate() {
-        k.s = dpp_rotate<kDppWaveRol1>(k.s);
-        k.c = dpp_rotate<kDppWaveRol1>(k.c);
+    float s = 0.0f;
+    float c = 0.0f;
+    float g = 0.0f;
+    __device__ __forceinline__ void add(float v) { g += v; }
+    __device__ __forceinline__ void add_product(float a, float b) { g = fmaf(a, b, g); }
+    __device__ __forceinline__ void rotate() { g = dpp_rotate<kDppWaveRol1>(g); }
+    template <int STEPS>
+    __device__ __forceinline__ void flush() {
+        s = lane_rotate<STEPS>(s);
+        c = lane_rotate<STEPS>(c);
+        kahan_add(s, c, g);
+        g = 0.0f;
     }
-    __device__ __forceinline__ double value() const { return k.value(); }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
 }  // namespace mdship

@@ -187,6 +187,13 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
         load_group(bufA, bufAT, k0 + 2 * G);
         __builtin_amdgcn_sched_barrier(0);
         run_group(bufB, bufBT, k0 + G);
+        // close the group of 2 G steps (FP32: one compensated update per sum)
+        acc.s.flush();
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            acc.row[p].flush();
+            acc.col[p].template flush<2 * G>();
+        }
     }
 
     const double s = wave_sum_to_last(acc.s.value());
