@@ -65,38 +65,45 @@ constexpr int kBatchMaxThreads = 512;
 
 template <typename Real, int P>
 __device__ __forceinline__ void batch_tile(const BatchDataset& d, uint32_t t, double* __restrict__ tileSums,
-                                           const Real* lnPhiTab) {
+                                           const Real* lnPhiTab, Real* stage) {
     const uint32_t code = d.tiles[t];
     const int I = (int)(code >> 16), J = (int)(code & 0xffffu);
     const Real* base = (const Real*)d.delta + (int64_t)t * kTileElems;
     const Real* x = (const Real*)d.x;
     const PairConst<Real> pc = make_pair_const<Real>(d.tau, lnPhiTab);
     double* sum = tileSums + d.tileSumBase + t;
+    // the previous tile's reads of the staged columns precede the new writes
+    wave_lds_sync();
+    stage_columns<Real, P>(stage, x, J);
+    wave_lds_sync();
     if (d.truncated) {
         if (d.symmetric)
-            tile_body<Real, P, true, true, true, kGroup>(base, d.planeStride, x, I, J, d.nb, pc, sum, d.part);
+            tile_walk<Real, P, true, true, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum, d.part);
         else
-            tile_body<Real, P, true, false, true, kGroup>(base, d.planeStride, x, I, J, d.nb, pc, sum, d.part);
+            tile_walk<Real, P, true, false, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum,
+                                                          d.part);
     } else {
         if (d.symmetric)
-            tile_body<Real, P, false, true, true, kGroup>(base, d.planeStride, x, I, J, d.nb, pc, sum, d.part);
+            tile_walk<Real, P, false, true, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum,
+                                                          d.part);
         else
-            tile_body<Real, P, false, false, true, kGroup>(base, d.planeStride, x, I, J, d.nb, pc, sum, d.part);
+            tile_walk<Real, P, false, false, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum,
+                                                           d.part);
     }
 }
 
 template <typename Real>
 __device__ __forceinline__ void batch_tile_any(const BatchDataset& d, uint32_t t, double* __restrict__ tileSums,
-                                               const Real* lnPhiTab) {
+                                               const Real* lnPhiTab, Real* stage) {
     switch (d.kdim) {
-        case 1: batch_tile<Real, 1>(d, t, tileSums, lnPhiTab); break;
-        case 2: batch_tile<Real, 2>(d, t, tileSums, lnPhiTab); break;
-        case 3: batch_tile<Real, 3>(d, t, tileSums, lnPhiTab); break;
-        case 4: batch_tile<Real, 4>(d, t, tileSums, lnPhiTab); break;
-        case 5: batch_tile<Real, 5>(d, t, tileSums, lnPhiTab); break;
-        case 6: batch_tile<Real, 6>(d, t, tileSums, lnPhiTab); break;
-        case 8: batch_tile<Real, 8>(d, t, tileSums, lnPhiTab); break;
-        case 10: batch_tile<Real, 10>(d, t, tileSums, lnPhiTab); break;
+        case 1: batch_tile<Real, 1>(d, t, tileSums, lnPhiTab, stage); break;
+        case 2: batch_tile<Real, 2>(d, t, tileSums, lnPhiTab, stage); break;
+        case 3: batch_tile<Real, 3>(d, t, tileSums, lnPhiTab, stage); break;
+        case 4: batch_tile<Real, 4>(d, t, tileSums, lnPhiTab, stage); break;
+        case 5: batch_tile<Real, 5>(d, t, tileSums, lnPhiTab, stage); break;
+        case 6: batch_tile<Real, 6>(d, t, tileSums, lnPhiTab, stage); break;
+        case 8: batch_tile<Real, 8>(d, t, tileSums, lnPhiTab, stage); break;
+        case 10: batch_tile<Real, 10>(d, t, tileSums, lnPhiTab, stage); break;
         default: break;  // the host only admits kernel dimensions <= 10 here
     }
 }
@@ -109,7 +116,10 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batch_tile_kernel(const Batc
                                                                        int numItems,
                                                                        double* __restrict__ tileSums) {
     __shared__ __align__(16) Real lnPhiTab[TableOf<Real>::kCount];  // truncated datasets read it
-    load_lnphi_table(lnPhiTab);
+    __shared__ __align__(16) Real colStage[kBatchMaxThreads / kWave][ColumnStage<Real, 1>::kMaxElems];
+    copy_lnphi_table(lnPhiTab);
+    __syncthreads();
+    Real* stage = colStage[threadIdx.x / kWave];
     const int wavesPerBlock = blockDim.x / kWave;
     const int stride = gridDim.x * wavesPerBlock;
     for (int w = __builtin_amdgcn_readfirstlane(blockIdx.x * wavesPerBlock + threadIdx.x / kWave); w < numItems;
@@ -118,9 +128,9 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batch_tile_kernel(const Batc
         const BatchDataset d = ds[it.ds];
         for (uint32_t c = 0; c < it.count; ++c) {
             if constexpr (KDIM == 0)
-                batch_tile_any<Real>(d, it.tile + c, tileSums, lnPhiTab);
+                batch_tile_any<Real>(d, it.tile + c, tileSums, lnPhiTab, stage);
             else
-                batch_tile<Real, KDIM>(d, it.tile + c, tileSums, lnPhiTab);
+                batch_tile<Real, KDIM>(d, it.tile + c, tileSums, lnPhiTab, stage);
         }
     }
 }

@@ -156,6 +156,14 @@ __device__ __forceinline__ float lane_rotate(float v) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
 }
 
+template <int SHIFT>
+__device__ __forceinline__ double lane_rotate(double v) {
+    const int src = (((int)(threadIdx.x % kWave) + SHIFT) & (kWave - 1)) << 2;
+    const int lo = __builtin_amdgcn_ds_bpermute(src, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(src, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
 template <>
 struct ColSum<float> {
     float s = 0.0f;

@@ -11,8 +11,10 @@
 // every transcendental is evaluated once per unordered pair.
 //
 // One wavefront owns one tile: lane l is row i = 64 I + l and visits the 64
-// columns in 64 steps.  Row partials of the gradient stay in registers;
-// column partials rotate across the wavefront with DPP.
+// columns in 64 steps (diagonal tiles: 32 steps, each unordered pair once).
+// Row partials of the gradient stay in registers; column partials rotate
+// across the wavefront with DPP; column locations are read from LDS (small
+// P) or rotate with the partials.
 // Each tile writes its likelihood partial and its two 64-row gradient
 // partials to fixed slots; a second kernel adds them in a fixed order in
 // FP64.  No atomics anywhere, so results are bitwise reproducible.
@@ -22,20 +24,25 @@
 namespace mdship {
 
 // Single instance: one wavefront per tile of the upper block triangle.  The
-// truncated forms first copy the log Phi table into LDS (every wavefront of
-// the block takes part, before any leaves).
+// truncated forms copy the log Phi table into LDS and every wavefront stages
+// its tile's column locations (small P); all wavefronts of the block take
+// part in the barrier before any leaves.
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
     const Real* __restrict__ delta, const Real* __restrict__ x,
     const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
     double tau, double* __restrict__ tileSums, double* __restrict__ part, const double* __restrict__ devTau) {
     __shared__ __align__(16) Real lnPhiTab[TRUNC ? TableOf<Real>::kCount : 1];
-    if constexpr (TRUNC) load_lnphi_table(lnPhiTab);
+    __shared__ __align__(16) Real colStage[kWavesPerBlock][ColumnStage<Real, P>::kElems];
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
+    const uint32_t code = t < numTiles ? tiles[t] : 0u;
+    Real* stage = colStage[threadIdx.x / kWave];
+    if (t < numTiles) stage_columns<Real, P>(stage, x, (int)(code & 0xffffu));
+    if constexpr (TRUNC) copy_lnphi_table(lnPhiTab);
+    __syncthreads();
     if (t >= numTiles) return;
     const PairConst<Real> pc = make_pair_const<Real>(devTau ? *devTau : tau, lnPhiTab);
-    const uint32_t code = tiles[t];
-    tile_body<Real, P, TRUNC, SYM, GRAD, kGroup>(delta + (int64_t)t * kTileElems, planeStride, x,
+    tile_walk<Real, P, TRUNC, SYM, GRAD, kGroup>(delta + (int64_t)t * kTileElems, planeStride, x, stage,
                                                  (int)(code >> 16), (int)(code & 0xffffu), nb, pc, tileSums + t,
                                                  part);
 }

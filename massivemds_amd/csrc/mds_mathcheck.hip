@@ -22,6 +22,9 @@ double cut26(double v) {
 struct SoftSeeds {
     static double rsq(double x) { return cut26(1.0 / std::sqrt(x)); }
     static float rsq(float x) { return 1.0f / std::sqrt(x); }
+    static double fract(double x) { return x - std::floor(x); }
+    static float fract(float x) { return x - std::floor(x); }
+    static int mul24(int a, int b) { return a * b; }
 };
 }  // namespace
 
@@ -36,7 +39,7 @@ void mds_mathcheck_f64(const double* d2, long long n, double tau, double* d, dou
     const double gScale = mdship::coeffs::kLnPhiTabStep / sqrtTau;
     for (long long k = 0; k < n; ++k) {
         mdship::distance64<SoftSeeds, 1>(d2[k], d[k], invd[k]);
-        const mdship::Trunc<double> t = mdship::trunc64_table(d[k], sT, mdship::coeffs::kLnPhiTab);
+        const mdship::Trunc<double> t = mdship::trunc64_table<SoftSeeds>(d[k], sT, mdship::coeffs::kLnPhiTab);
         logPhi[k] = t.logPhi;
         gradTerm[k] = t.gradTerm * gScale;
     }
@@ -51,7 +54,7 @@ void mds_mathcheck_f32(const double* d2, long long n, double tau, double* d, dou
         float df, idf;
         const float d2f = (float)d2[k];
         mdship::distance32<SoftSeeds>(d2f, df, idf);
-        const mdship::Trunc<float> t = mdship::trunc32_table(df, sT, mdship::coeffs::kLnPhiTab32);
+        const mdship::Trunc<float> t = mdship::trunc32_table<SoftSeeds>(df, sT, mdship::coeffs::kLnPhiTab32);
         d[k] = df;
         invd[k] = idf;
         logPhi[k] = t.logPhi;

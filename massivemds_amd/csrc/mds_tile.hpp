@@ -9,14 +9,14 @@ namespace mdship {
 // Steps per prefetch group of the tile walk.
 constexpr int kGroup = 4;
 
-// Per-evaluation constants of the pair arithmetic: tau, tau / 2, and
-// sT = S sqrt(tau), z in the units of the log Phi table (S = 64 for FP64,
-// 32 for FP32), which `tab` points to in LDS.  The truncation's gradient
-// term g = d log Phi / d(S z) enters the pair coefficient as
+// Per-evaluation constants of the pair arithmetic: tau / 2, sT = S sqrt(tau)
+// (z in the units of the log Phi table, S = 64 for FP64 and 32 for FP32,
+// which `tab` points to in LDS) and sT / 2.  The truncation's gradient term
+// g = d log Phi / d(S z) enters the pair coefficient as
 // tau (phi / Phi) / sqrt(tau) = sT g.
 template <typename Real>
 struct PairConst {
-    Real tau, halfTau, sT;
+    Real halfTau, sT, halfST;
     const Real* tab;
 };
 
@@ -39,16 +39,15 @@ struct TableOf<float> {
 // read from a sampler's device parameter block), computed in FP64.
 template <typename Real>
 __device__ __forceinline__ PairConst<Real> make_pair_const(double tau, const Real* tab) {
-    return {(Real)tau, (Real)(0.5 * tau), (Real)(TableOf<Real>::kStep * sqrt(tau)), tab};
+    const double sT = TableOf<Real>::kStep * sqrt(tau);
+    return {(Real)(0.5 * tau), (Real)sT, (Real)(0.5 * sT), tab};
 }
 
 // Copies the log Phi table of this precision into LDS with every thread of
-// the block and waits for it; called before any wavefront of the block
-// leaves.
+// the block; the caller waits for it (__syncthreads) before any read.
 template <typename Real>
-__device__ __forceinline__ void load_lnphi_table(Real* lds) {
+__device__ __forceinline__ void copy_lnphi_table(Real* lds) {
     for (int i = threadIdx.x; i < TableOf<Real>::kCount; i += blockDim.x) lds[i] = TableOf<Real>::value(i);
-    __syncthreads();
 }
 
 __device__ __forceinline__ Trunc<double> trunc_terms(double d, const PairConst<double>& c) {
@@ -64,6 +63,44 @@ __device__ __forceinline__ void distance(double d2, double& d, double& invd) {
 }
 __device__ __forceinline__ void distance(float d2, float& d, float& invd) { distance32<HwSeeds>(d2, d, invd); }
 
+// Column locations of a tile.  When a column's P coordinates fit in 16
+// bytes (FP64 P <= 2, FP32 P <= 4) each wavefront stages its tile's 64
+// columns in LDS twice over, coordinate-major ([P][128]): step k of lane l
+// reads word l + k, so the step is an immediate offset and no lane index
+// wraps.  Larger P keep the columns in registers and rotate them one lane
+// per step (DPP), which costs no LDS and leaves occupancy alone.
+template <typename Real, int P>
+struct ColumnStage {
+    static constexpr bool kInLds = sizeof(Real) * P <= 16;
+    static constexpr int kElems = kInLds ? P * 2 * kTile : 1;  // per wavefront
+    static constexpr int kMaxElems = 16 / sizeof(Real) * 2 * kTile;  // any P that stages
+};
+
+// Lane l writes column 64 J + l of x ([nb * 64][P]) into both halves.
+template <typename Real, int P>
+__device__ __forceinline__ void stage_columns(Real* __restrict__ stage, const Real* __restrict__ x, int J) {
+    if constexpr (ColumnStage<Real, P>::kInLds) {
+        const int lane = threadIdx.x % kWave;
+        const Real* src = x + ((int64_t)J * kTile + lane) * P;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const Real v = src[p];
+            stage[p * 2 * kTile + lane] = v;
+            stage[p * 2 * kTile + kTile + lane] = v;
+        }
+    }
+}
+
+// Orders one wavefront's LDS accesses across a point where lanes start to
+// read what other lanes wrote (or overwrite what they read): LDS executes a
+// wavefront's instructions in order, so only the compiler must not move
+// accesses across it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Per-wavefront accumulators of one tile.
 template <typename Real, int P>
 struct TileAcc {
@@ -73,11 +110,14 @@ struct TileAcc {
 };
 
 // One step of the skewed schedule for one lane: the pair (row i, column c)
-// with observations dij = delta_ij and dji = delta_ji.
+// with observations dij = delta_ij and dji = delta_ji.  The gradient
+// accumulators take half the pair coefficients (tau / 2 and sT / 2 in place
+// of tau and sT: exact halvings), and the tile doubles its partials at the
+// end, so that tau / 2 serves the likelihood and the gradient alike.
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi)[P], const Real (&xj)[P],
                                           Real dij, Real dji, bool inTri, const PairConst<Real>& pc) {
-    const Real tau = pc.tau, halfTau = pc.halfTau;
+    const Real halfTau = pc.halfTau;
     const bool vij = inTri && !m_isnan(dij);
     const bool vji = inTri && !m_isnan(dji);
     Real diff[P];
@@ -91,13 +131,15 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
     distance(d2, d, invd);
     const Real rij = dij - d;
     const Real rji = dji - d;
+    const Real aij = halfTau * rij;
+    const Real aji = halfTau * rji;
 
     Real lij, lji;
     Real gt = Real(0);
     if (TRUNC) {
         const Trunc<Real> tt = trunc_terms(d, pc);
-        lij = halfTau * rij * rij + tt.logPhi;
-        lji = halfTau * rji * rji + tt.logPhi;
+        lij = fma(aij, rij, tt.logPhi);
+        lji = fma(aji, rji, tt.logPhi);
         gt = tt.gradTerm;
     } else {
         lij = rij * rij;
@@ -111,15 +153,14 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
         acc.s.add((vij ? lij : Real(0)) + (vji ? lji : Real(0)));
 
     if (GRAD) {
-        // c = (r - phi / (Phi sqrt tau)) * tau / d for each ordered pair
-        // (src/NewMultiDimensionalScaling.hpp:435-444); masked pairs give 0.
-        const Real cij = vij ? (TRUNC ? fma(-gt, pc.sT, tau * rij) : tau * rij) * invd : Real(0);
-        const Real cji =
-            SYM ? cij : (vji ? (TRUNC ? fma(-gt, pc.sT, tau * rji) : tau * rji) * invd : Real(0));
+        // c / 2 = (r - phi / (Phi sqrt tau)) * (tau / 2) / d for each ordered
+        // pair (src/NewMultiDimensionalScaling.hpp:435-444); masked pairs give 0.
+        const Real hij = vij ? (TRUNC ? fma(-gt, pc.halfST, aij) : aij) * invd : Real(0);
+        const Real hji = SYM ? hij : (vji ? (TRUNC ? fma(-gt, pc.halfST, aji) : aji) * invd : Real(0));
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            acc.row[p].add_product(cij, diff[p]);
-            acc.col[p].add_product(SYM ? -cij : -cji, diff[p]);
+            acc.row[p].add_product(hij, diff[p]);
+            acc.col[p].add_product(SYM ? -hij : -hji, diff[p]);
             acc.col[p].rotate();
         }
     }
@@ -128,20 +169,32 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
 // One wavefront per tile.  Skewed schedule: at step k lane l evaluates the
 // pair (row 64 I + l, column 64 J + (l + k) mod 64); the engine stores each
 // tile in that order, so every step is one coalesced 64-value load.  Row
-// partials stay in the lane; column partials and the column locations
-// rotate one lane per step (column partials end in the lane of their
-// column).  Observations are prefetched one group of G steps ahead.
+// partials stay in the lane; column partials rotate one lane per step (DPP)
+// and end in the lane of their column.  Observations are prefetched one
+// group of G steps ahead.
 //
-// tileBase: this tile's plane-0 values; x: [nb * 64][P] locations; writes
-// *tileSum (lane 63) and the two gradient slots of part ([nb][nb][P][64]).
-template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, int G>
+// Off-diagonal tiles (DIAG = false) take steps 0..63 with no triangle mask.
+// A diagonal tile holds each unordered pair {a, b} of its block twice; steps
+// 1..31 with every lane and step 32 with lanes 0..31 visit each exactly
+// once (lane a at step b - a when b - a <= 32, else lane b at step
+// 64 - (b - a), as the row of the pair with the observations read in that
+// orientation), half the steps of the full walk.
+//
+// tileBase: this tile's plane-0 values; x: [nb * 64][P] locations; stage:
+// this wavefront's staged columns (see ColumnStage; unused for larger P);
+// writes *tileSum (lane 63) and the two gradient slots of part
+// ([nb][nb][P][64]).
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, bool DIAG, int G>
 __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int64_t planeStride,
-                                          const Real* __restrict__ x, int I, int J, int nb,
-                                          const PairConst<Real>& pc, double* __restrict__ tileSum,
+                                          const Real* __restrict__ x, const Real* __restrict__ stage, int I,
+                                          int J, int nb, const PairConst<Real>& pc, double* __restrict__ tileSum,
                                           double* __restrict__ part) {
-    static_assert(kTile % (2 * G) == 0, "group size must divide half a tile");
+    static_assert(kTile % (2 * G) == 0 && (kTile / 2) % (2 * G) == 0, "group size must divide a quarter tile");
+    constexpr bool kStaged = ColumnStage<Real, P>::kInLds;
+    constexpr int kFirst = DIAG ? 1 : 0;                 // first step
+    constexpr int kSteps = DIAG ? kTile / 2 : kTile;     // steps walked
+    constexpr int kLast = kFirst + kSteps - 1;
     const int lane = threadIdx.x % kWave;
-    const bool diag = (I == J);
     const Real* tileStep = tileBase + lane;
     // Two register buffers of G steps each (ping-pong): while one group is
     // evaluated the next one is in flight.  The loads of a group are pinned
@@ -151,7 +204,7 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
     auto load_group = [&](Real (&b)[G], Real (&bT)[G], int k0) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            const int kn = min(k0 + g, kTile - 1);  // past the end: re-read step 63
+            const int kn = min(k0 + g, kLast);  // past the walk: re-read its last step
             b[g] = tileStep[kn * kTile];
             bT[g] = SYM ? b[g] : tileStep[planeStride + kn * kTile];
         }
@@ -159,28 +212,39 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
 
     Real xi[P], xj[P];
     const Real* xrow = x + ((int64_t)I * kTile + lane) * P;
-    const Real* xcol = x + ((int64_t)J * kTile + lane) * P;  // column (lane + 0) mod 64
+    const Real* xcol = x + ((int64_t)J * kTile + ((lane + kFirst) & (kTile - 1))) * P;  // column of step kFirst
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         xi[p] = xrow[p];
-        xj[p] = xcol[p];
+        if constexpr (!kStaged) xj[p] = xcol[p];
     }
+    const Real* stageLane = stage + lane;
 
     TileAcc<Real, P> acc;
     auto run_group = [&](const Real (&b)[G], const Real (&bT)[G], int k0) {
+        Real xs[G][P];
+        if constexpr (kStaged) {
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int p = 0; p < P; ++p) xs[g][p] = stageLane[p * 2 * kTile + k0 + g];
+        }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int k = k0 + g;
-            // diagonal tiles keep i < j only: lane < (lane + k) mod 64
-            const bool inTri = !diag || (k > 0 && lane + k < kTile);
-            pair_step<Real, P, TRUNC, SYM, GRAD>(acc, xi, xj, b[g], bT[g], inTri, pc);
+            const bool inTri = !DIAG || k < kTile / 2 || lane < kTile / 2;
+            if constexpr (kStaged) {
+                pair_step<Real, P, TRUNC, SYM, GRAD>(acc, xi, xs[g], b[g], bT[g], inTri, pc);
+            } else {
+                pair_step<Real, P, TRUNC, SYM, GRAD>(acc, xi, xj, b[g], bT[g], inTri, pc);
 #pragma unroll
-            for (int p = 0; p < P; ++p) xj[p] = dpp_rotate<kDppWaveRol1>(xj[p]);
+                for (int p = 0; p < P; ++p) xj[p] = dpp_rotate<kDppWaveRol1>(xj[p]);
+            }
         }
     };
 
-    load_group(bufA, bufAT, 0);
-    for (int k0 = 0; k0 < kTile; k0 += 2 * G) {
+    load_group(bufA, bufAT, kFirst);
+    for (int k0 = kFirst; k0 <= kLast; k0 += 2 * G) {
         load_group(bufB, bufBT, k0 + G);
         __builtin_amdgcn_sched_barrier(0);
         run_group(bufA, bufAT, k0);
@@ -200,19 +264,36 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
     if (lane == kWave - 1) *tileSum = SYM ? 2.0 * s : s;
 
     if (GRAD) {
+        // the accumulators hold half the gradient terms: doubled here (exact)
         double* rowSlot = part + ((int64_t)I * nb + J) * (P * kTile) + lane;
-        if (diag) {
+        if constexpr (DIAG) {
+            // after the walk lane l holds column (l + kFirst + kSteps) mod 64
+            constexpr int kAlign = (2 * kTile - kFirst - kSteps) % kTile;
 #pragma unroll
-            for (int p = 0; p < P; ++p) rowSlot[p * kTile] = acc.row[p].value() + acc.col[p].value();
+            for (int p = 0; p < P; ++p)
+                rowSlot[p * kTile] = 2.0 * (acc.row[p].value() + lane_rotate<kAlign>(acc.col[p].value()));
         } else {
             double* colSlot = part + ((int64_t)J * nb + I) * (P * kTile) + lane;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                rowSlot[p * kTile] = acc.row[p].value();
-                colSlot[p * kTile] = acc.col[p].value();
+                rowSlot[p * kTile] = 2.0 * acc.row[p].value();
+                colSlot[p * kTile] = 2.0 * acc.col[p].value();
             }
         }
     }
+}
+
+// Both walks of the tile (I, J): diagonal tiles take the half walk.
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, int G>
+__device__ __forceinline__ void tile_walk(const Real* __restrict__ tileBase, int64_t planeStride,
+                                          const Real* __restrict__ x, const Real* __restrict__ stage, int I,
+                                          int J, int nb, const PairConst<Real>& pc, double* __restrict__ tileSum,
+                                          double* __restrict__ part) {
+    if (I == J)
+        tile_body<Real, P, TRUNC, SYM, GRAD, true, G>(tileBase, planeStride, x, stage, I, J, nb, pc, tileSum, part);
+    else
+        tile_body<Real, P, TRUNC, SYM, GRAD, false, G>(tileBase, planeStride, x, stage, I, J, nb, pc, tileSum,
+                                                        part);
 }
 
 }  // namespace mdship

@@ -19,9 +19,10 @@
 //
 // The kernels hold the tables in LDS; gen_trunc_coeffs.py fits them and
 // reports the errors.  d and 1/d come from the hardware reciprocal square
-// root (FP64: one Newton step).  The `Seeds` policy supplies that seed: the
-// gfx950 instruction on the device, a stand-in of the same accuracy for the
-// host build that checks the arithmetic on the CPU.
+// root (FP64: one Newton step).  The `Seeds` policy supplies that seed and the
+// fractional part: the gfx950 instructions on the device, stand-ins (of the
+// same accuracy for the seed, exact for the fractional part) for the host
+// build that checks the arithmetic on the CPU.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,13 +34,24 @@ namespace mdship {
 struct HwSeeds {
     __device__ static double rsq(double x) { return __builtin_amdgcn_rsq(x); }
     __device__ static float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+    // x - floor(x), exact for finite x >= 0 (one instruction)
+    __device__ static double fract(double x) { return __builtin_amdgcn_fract(x); }
+    __device__ static float fract(float x) { return __builtin_amdgcn_fractf(x); }
+    // a * b for 0 <= a, b < 2^23 (full-rate 24-bit multiply)
+    __device__ static int mul24(int a, int b) { return __mul24(a, b); }
 };
+
+// 2^500: d2 * kDistBig exceeds sqrt(d2) for every d2 > 2^-1000, and is 0 at
+// d2 = 0, where the Newton step gives NaN (0 * inf).
+constexpr double kDistBig = 0x1p500;
 
 // d = sqrt(d2) and 1/d.  d2 = 0 gives d = 0 and 1/d = NaN (there the
 // reference's gradient term (r / d) (x_i - x_j) is inf * 0 = NaN as well, and
-// masked pairs are zeroed after); a NaN d2 gives NaN for both.  The seed's
-// relative error e0 (v_rsq_f64: ~1.7e-8) becomes ~1.5 e0^2 after NEWTON = 1
-// step (~5e-16) and rounding level after 2.
+// masked pairs are zeroed after); a NaN d2 gives NaN for both, d2 = inf gives
+// d = inf.  The seed's relative error e0 (v_rsq_f64: ~1.7e-8) becomes
+// ~1.5 e0^2 after NEWTON = 1 step (~5e-16) and rounding level after 2.
+// The minimum with d2 * 2^500 (IEEE minNum: a NaN operand yields the other)
+// turns the NaN of d2 = 0 into 0 and leaves every d2 > 2^-1000 unchanged.
 template <class S, int NEWTON = 1>
 __host__ __device__ inline void distance64(double d2, double& d, double& invd) {
     double y = S::rsq(d2);
@@ -50,7 +62,7 @@ __host__ __device__ inline void distance64(double d2, double& d, double& invd) {
         y = fma(y, fma(-hy, y, 0.5), y);
     }
     invd = y;
-    d = d2 == 0.0 ? 0.0 : d2 * y;
+    d = fmin(d2 * y, d2 * kDistBig);
 }
 
 template <class S>
@@ -68,15 +80,16 @@ struct Trunc {
 
 // FP64.  sT = S sqrt(tau) (z in table units, zT = sT d); tab: kLnPhiTabRows
 // rows of kLnPhiTabStride values, 16-byte aligned.  zT is clamped to the zero
-// row, which also catches inf and NaN distances.
+// row, which also catches inf and NaN distances.  zT >= 0, so the row is the
+// truncation of zT and t its fractional part.
+template <class S = HwSeeds>
 __host__ __device__ inline Trunc<double> trunc64_table(double d, double sT, const double* __restrict__ tab) {
     constexpr int kPairs = coeffs::kLnPhiTabStride / 2;
     static_assert(coeffs::kLnPhiTabStride % 2 == 0 && coeffs::kLnPhiTabDeg < coeffs::kLnPhiTabStride,
                   "rows are read as 16-byte pairs");
     const double zT = fmin(d * sT, coeffs::kLnPhiTabClamp);
-    const double fl = floor(zT);
-    const double t = zT - fl;
-    const double2* row = reinterpret_cast<const double2*>(tab) + (int)(fl * kPairs);  // exact: fl < 2^20
+    const double t = S::fract(zT);
+    const double2* row = reinterpret_cast<const double2*>(tab) + S::mul24((int)zT, kPairs);
     double c[coeffs::kLnPhiTabStride];
 #pragma unroll
     for (int h = 0; h < kPairs; ++h) {
@@ -99,12 +112,12 @@ __host__ __device__ inline Trunc<double> trunc64_table(double d, double sT, cons
 }
 
 // FP32: one 16-byte row; sT = S sqrt(tau) with S = kLnPhiTab32Step.
+template <class S = HwSeeds>
 __host__ __device__ inline Trunc<float> trunc32_table(float d, float sT, const float* __restrict__ tab) {
     static_assert(coeffs::kLnPhiTab32Stride == 4 && coeffs::kLnPhiTab32Deg == 3, "one 16-byte row");
     const float zT = fminf(d * sT, coeffs::kLnPhiTab32Clamp);
-    const float fl = floorf(zT);
-    const float t = zT - fl;
-    const float4 c = reinterpret_cast<const float4*>(tab)[(int)fl];
+    const float t = S::fract(zT);
+    const float4 c = reinterpret_cast<const float4*>(tab)[(int)zT];
     float p = fmaf(c.w, t, c.z);
     float q = fmaf(c.w, t, p);
     p = fmaf(p, t, c.y);
