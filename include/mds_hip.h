@@ -168,8 +168,8 @@ int mds_hip_batch_get_timing(mds_hip_batch_handle h, double* ms, long long* coun
  * Device-resident HMC sampler: the GPU counterpart of the reference's R
  * hmcsampler (R/Untitled.R:477-729) for the model without a tree (row
  * covariance I_N): potential = -(matrix-normal log prior + MDS
- * log-likelihood), leapfrog with step trajectory / leapfrogSteps, one
- * hipGraph per leapfrog step, Metropolis test in FP64 on the host.
+ * log-likelihood), leapfrog with step trajectory / leapfrogSteps, the
+ * trajectory captured as one hipGraph, Metropolis test in FP64 on the host.
  * learnPrecision: random-walk Metropolis on tau with the reference proposal
  * |U(tau - w, tau + w)| under truncation, Gamma Gibbs draw otherwise.
  * learnTraitPrecision: Wishart(P + N, (I + X^T X)^-1) Gibbs draw every

@@ -8,9 +8,9 @@
 //   log-likelihood = (log tau - log 2 pi) N (N - 1) / 4 - sumOfIncrements
 //   (R/mds.R:24-28), momentum ~ N(0, I), leapfrog with step trajectory / L.
 //
-// Locations, momenta and gradients stay on the device; one leapfrog step is
-// captured once as a hipGraph (kick + drift, the fused likelihood/gradient
-// kernels, kick) and replayed L times per iteration.  The precision and the
+// Locations, momenta and gradients stay on the device; the trajectory of L
+// leapfrog steps (kicks and drifts, the fused likelihood/gradient kernels)
+// is captured once as a hipGraph and replayed once per iteration.  The precision and the
 // trait precision live in a device parameter block, so the graph stays valid
 // when they are updated.  The Hamiltonian and the Metropolis log-ratio are
 // formed on the host in FP64.
@@ -70,8 +70,15 @@ __global__ void momentum_kernel(double* __restrict__ mom, int count, uint64_t se
 }
 
 // Row i: g = grad log-likelihood + grad log prior = grad_ll - x_i T.
-// drift: mom += halfEps * g(x); x += eps * mom.  kick: mom += halfEps * g(x).
-template <typename Real, bool DRIFT>
+// The reference's leapfrog (R/Untitled.R:611-620) takes two half kicks per
+// step, mom += eps / 2 * g before and after the drift x += eps * mom; the
+// kernel forms: KICK mom += halfEps * g; KICK_DRIFT the kick, then the drift;
+// KICK2_DRIFT the half kick ending one step and the one starting the next
+// (same locations, so the same g; two separate additions, as two launches
+// would make them), then the drift.
+enum KickForm { kKick = 0, kKickDrift = 1, kKick2Drift = 2 };
+
+template <typename Real, int FORM>
 __global__ void kick_kernel(Real* __restrict__ x, double* __restrict__ mom, const double* __restrict__ grad,
                             const double* __restrict__ params, int n, int p, int kdim) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,9 +91,12 @@ __global__ void kick_kernel(Real* __restrict__ x, double* __restrict__ mom, cons
         double prior = 0.0;
         for (int e = 0; e < p; ++e) prior += (double)xi[e] * T[e * p + d];
         const double g = grad[(int64_t)i * p + d] - prior;
-        mom[(int64_t)i * p + d] += halfEps * g;
+        double m = mom[(int64_t)i * p + d];
+        m = fma(halfEps, g, m);
+        if (FORM == kKick2Drift) m = fma(halfEps, g, m);
+        mom[(int64_t)i * p + d] = m;
     }
-    if (DRIFT) {
+    if (FORM != kKick) {
         for (int d = 0; d < p; ++d) xi[d] = (Real)((double)xi[d] + eps * mom[(int64_t)i * p + d]);
     }
 }
@@ -287,25 +297,29 @@ void Hmc::uploadParams() {
           "hipMemcpy(parameters)");
 }
 
-void Hmc::leapfrogStep(int mode) {
+template <int FORM>
+void Hmc::kick() {
     const int threads = 256, blocks = (n + threads - 1) / threads;
-    if (e.fp32) {
-        hipLaunchKernelGGL((kick_kernel<float, true>), dim3(blocks), dim3(threads), 0, e.stream, (float*)e.dX[e.cur],
-                           dMom, e.dGrad, dParams, n, p, e.kdim);
-    } else {
-        hipLaunchKernelGGL((kick_kernel<double, true>), dim3(blocks), dim3(threads), 0, e.stream,
-                           (double*)e.dX[e.cur], dMom, e.dGrad, dParams, n, p, e.kdim);
-    }
-    check(hipGetLastError(), "kick/drift kernel");
-    gradient(mode);
-    if (e.fp32) {
-        hipLaunchKernelGGL((kick_kernel<float, false>), dim3(blocks), dim3(threads), 0, e.stream,
+    if (e.fp32)
+        hipLaunchKernelGGL((kick_kernel<float, FORM>), dim3(blocks), dim3(threads), 0, e.stream,
                            (float*)e.dX[e.cur], dMom, e.dGrad, dParams, n, p, e.kdim);
-    } else {
-        hipLaunchKernelGGL((kick_kernel<double, false>), dim3(blocks), dim3(threads), 0, e.stream,
+    else
+        hipLaunchKernelGGL((kick_kernel<double, FORM>), dim3(blocks), dim3(threads), 0, e.stream,
                            (double*)e.dX[e.cur], dMom, e.dGrad, dParams, n, p, e.kdim);
-    }
     check(hipGetLastError(), "kick kernel");
+}
+
+// One trajectory of opt.leapfrogSteps steps from the gradient now in
+// e.dGrad: kick + drift, then per step the gradient and the kicks.
+void Hmc::trajectory(int mode) {
+    kick<kKickDrift>();
+    for (int s = 0; s < opt.leapfrogSteps; ++s) {
+        gradient(mode);
+        if (s + 1 < opt.leapfrogSteps)
+            kick<kKick2Drift>();
+        else
+            kick<kKick>();
+    }
 }
 
 void Hmc::gradientAtCurrent() { e.evaluate(true, dParams + kHmcTau); }
@@ -326,12 +340,13 @@ void Hmc::gradient(int mode) {
         launch_snapshot(e, e.dGrad, sur->store.sx, sur->store.sg, sur->store.dSlot, sur->store.capacity, e.stream);
 }
 
+// The whole trajectory as one graph: one launch per iteration.
 void Hmc::buildGraph(int mode) {
     if (graphExec[mode]) return;
     const bool timing = e.timing;
     e.timing = false;
     check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    leapfrogStep(mode);
+    trajectory(mode);
     check(hipStreamEndCapture(e.stream, &graph[mode]), "hipStreamEndCapture");
     check(hipGraphInstantiate(&graphExec[mode], graph[mode], nullptr, nullptr, 0), "hipGraphInstantiate");
     e.timing = timing;
@@ -452,12 +467,10 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
         if (mode == kExactCollect)  // snapshots of a rejected trajectory are dropped
             check(hipMemcpyAsync(sur->store.dSlot + 1, sur->store.dSlot, sizeof(double), hipMemcpyDeviceToDevice,
                                  e.stream), "hipMemcpy(slot)");
-        for (int s = 0; s < opt.leapfrogSteps; ++s) {
-            if (graphExec[mode])
-                check(hipGraphLaunch(graphExec[mode], e.stream), "hipGraphLaunch");
-            else
-                leapfrogStep(mode);
-        }
+        if (graphExec[mode])
+            check(hipGraphLaunch(graphExec[mode], e.stream), "hipGraphLaunch");
+        else
+            trajectory(mode);
         if (mode == kSurrogate) {
             // the exact likelihood (FP64-reduced) judges the proposal
             e.evaluate(false, dParams + kHmcTau);

@@ -39,7 +39,9 @@ struct Hmc {
     Hmc(Engine& engine, const mds_hip_hmc_options& options);
     ~Hmc();
     void uploadParams();
-    void leapfrogStep(int mode);
+    template <int FORM>
+    void kick();
+    void trajectory(int mode);
     void gradientAtCurrent();
     void gradient(int mode);
     void buildGraph(int mode);
