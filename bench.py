@@ -254,7 +254,7 @@ def _bench_batch(steps, warmup, count=64, tune_cache=None):
 def _bench_hmc(iterations=2000, burn_in=500, n=5000, p=2):
     """Plain HMC on the GPU: N = 5000, P = 2, FP64, euclid10 with 2% missing,
     truncation, 20 leapfrog steps over trajectory 0.1, tau and the trait
-    precision learned, one hipGraph per leapfrog step."""
+    precision learned, each trajectory one hipGraph launch."""
     from massivemds_amd import datagen, hmc
 
     delta = datagen.euclid10(n, missing=0.02)
