@@ -387,6 +387,78 @@ def _bench_beast(steps=2000, warmup=20):
     print(json.dumps(out))
 
 
+def _bench_reference_loop(iterations=100, n=1000, p=2, tau=1.0, no_cpu=False, cpu_budget=10.0):
+    """The reference benchmark's loop (src/benchmark.cpp:248-301) on the
+    reference's own CPU-sized case: N = 1000, P = 2, FP64, euclid10
+    observations without noise or missing pairs, tau = 1, truncation; every
+    iteration draws fresh locations (src/benchmark.cpp:257), uploads them and
+    reads back the sum of increments and the gradient through the API.
+    Reported: that API rate, the device-resident rate of the same evaluation,
+    and the CPU port (serial and all-core SIMD) on the first iterations'
+    locations, with the GPU's relative error against it."""
+    from massivemds_amd import datagen
+    from massivemds_amd.engine import Flags, MultiDimensionalScaling
+
+    delta = datagen.euclid10(n)
+    xs = [datagen.locations(n, p, key=1 + k) for k in range(iterations)]
+    e = MultiDimensionalScaling(p, n, Flags.OPENCL | Flags.LEFT_TRUNCATION, 0)
+    e.setPairwiseData(delta)
+    e.setParameters([tau])
+    sums, grads = [], []
+    for k in range(3):  # warm-up
+        e.updateLocations(-1, xs[k])
+        e.getSumOfIncrements()
+        e.getLogLikelihoodGradient()
+    t0 = time.perf_counter()
+    for k in range(iterations):
+        e.updateLocations(-1, xs[k])
+        sums.append(e.getSumOfIncrements())
+        grads.append(e.getLogLikelihoodGradient())
+    api_s = time.perf_counter() - t0
+    e.close()
+    resident = _run_gpu(n, p, tau, delta, xs[0], False, 200, 10)
+    out = {
+        "workload": "mds_reference_loop_n1000_p2_trunc_tau1",
+        "N": n, "P": p, "precision": tau, "iterations": iterations,
+        "api_evals_per_s": iterations / api_s,
+        "api_us_per_eval": api_s / iterations * 1e6,
+        "resident_evals_per_s": 1e3 / resident["wall_ms_per_step"],
+        "tile_kernel_us": resident["tile_ms"] * 1e3,
+        "data": "synthetic (euclid10 distances, no noise, no missing; fresh N(0, I_2) locations per iteration)",
+    }
+    if not no_cpu:
+        from oracle import oracle
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or oracle.max_threads()
+        ser, allc, errs_s, errs_g = [], [], [], []
+        k = 0
+        while k < iterations and (sum(ser) + sum(allc) < cpu_budget or k < 2):
+            t0 = time.perf_counter()
+            s_ref = oracle.sum_of_increments(delta, xs[k], tau, True)
+            g_ref = oracle.gradient(delta, xs[k], tau, True)
+            ser.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            oracle.simd_sum_of_increments(delta, xs[k], tau, True, threads=threads)
+            oracle.simd_gradient(delta, xs[k], tau, True, threads=threads)
+            allc.append(time.perf_counter() - t0)
+            errs_s.append(abs(sums[k] - s_ref) / abs(s_ref))
+            errs_g.append(float(np.max(np.abs(grads[k].reshape(n, p) - g_ref.reshape(n, p)))
+                                / np.max(np.abs(g_ref))))
+            k += 1
+        out["cpu_baseline"] = {"value": 1.0 / float(np.median(allc)), "unit": "evals/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"the first {k} iterations' evaluations (likelihood pass + gradient "
+                                         f"pass), median; SIMD + OpenMP, {threads} threads"}
+        out["cpu_serial_1core"] = {"value": 1.0 / float(np.median(ser)), "unit": "evals/s", "cores": 1,
+                                   "sample": f"the first {k} iterations, median; the serial restatement"}
+        out["cpu_host"] = _cpu_host()
+        out["gpu_vs_cpu"] = {"api_speedup_vs_allcores": out["api_evals_per_s"] * float(np.median(allc)),
+                             "api_speedup_vs_serial": out["api_evals_per_s"] * float(np.median(ser)),
+                             "max_rel_err_sum": max(errs_s), "max_rel_err_grad": max(errs_g),
+                             "iterations_checked": k}
+    print(json.dumps(out))
+
+
 def _calibrate_copy(mib):
     """A plain elementwise copy kernel of known size (torch, as plumbing)."""
     import torch
@@ -409,8 +481,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--workload", default="n10000",
-                    choices=["n10000", "batch64", "hmc5000", "surrogate20000", "rowupdate10000", "beast372"],
-                    help="n10000: the metric's workload (default); batch64: the uneven batch; "
+                    choices=["n10000", "n1000", "batch64", "hmc5000", "surrogate20000", "rowupdate10000",
+                             "beast372"],
+                    help="n10000: the metric's workload (default); n1000: the reference benchmark's loop "
+                         "at N = 1000 (fresh locations per iteration, through the API); batch64: the uneven batch; "
                          "hmc5000: plain HMC; surrogate20000: surrogate against plain HMC; "
                          "rowupdate10000: single-location moves through the O(N) update; "
                          "beast372: real BEAST data (N = 372) against the logged likelihood")
@@ -422,6 +496,9 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         _calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload == "n1000":
+        _bench_reference_loop(no_cpu=args.no_cpu)
         return
     if args.workload == "batch64":
         _bench_batch(args.steps, args.warmup, tune_cache=args.tune_cache)
