@@ -68,6 +68,22 @@ struct Engine {
     bool devSumFresh = false;  // dSum matches the current state
     bool gradKnown = false;    // dGrad matches the current state
 
+    // Sum-then-gradient prediction.  The reference's callers mostly ask for
+    // the gradient right after the sum of the same state (the benchmark with
+    // --gradient, src/benchmark.cpp:262-270; the R Potential; BEAST's HMC),
+    // which as two API calls would run a likelihood pass and then the fused
+    // pass.  Once a likelihood-only evaluation has been followed by a
+    // gradient request, getSumOfIncrements runs the fused kernels instead and
+    // reads the gradient back with the sum; a fused evaluation whose gradient
+    // then goes unused turns the prediction off again.  Both kernels compute
+    // the sum in the same operations and order, so the sum is bitwise the
+    // same either way; only the work per call changes.
+    enum class StateEval { None, SumOnly, Fused };
+    bool predictGradient = false;
+    StateEval stateEval = StateEval::None;  // how the current state's sum was evaluated
+    bool gradRequested = false;             // the current state's gradient was asked for
+    bool hostGradFresh = false;             // hGrad holds the current state's gradient
+
     // single-row updates: the known sum of the state before row pendingRow
     // moved (its old location in dOldRow) is baseSum
     static constexpr int kMaxRowUpdates = 1024;
@@ -188,6 +204,10 @@ struct Engine {
     }
 
     void invalidate_device_results() {
+        if (stateEval == StateEval::Fused && !gradRequested) predictGradient = false;
+        stateEval = StateEval::None;
+        gradRequested = false;
+        hostGradFresh = false;
         devSumFresh = false;
         gradKnown = false;
     }
@@ -416,6 +436,7 @@ struct Engine {
         }
         devSumFresh = true;
         if (gradient) gradKnown = true;
+        hostGradFresh = false;
         ++fullEvaluations;
     }
 
@@ -425,12 +446,23 @@ struct Engine {
                 applyRowUpdate();
                 pendingRow = -1;
             } else {
-                if (!devSumFresh) evaluate(false);
+                bool gradCopied = false;
+                if (!devSumFresh) {
+                    evaluate(predictGradient);
+                    stateEval = predictGradient ? StateEval::Fused : StateEval::SumOnly;
+                    if (predictGradient) {
+                        check(hipMemcpyAsync(hGrad, dGrad, (size_t)N * P * sizeof(double), hipMemcpyDeviceToHost,
+                                             stream),
+                              "hipMemcpy(gradient)");
+                        gradCopied = true;
+                    }
+                }
                 check(hipMemcpyAsync(hSum, dSum, sizeof(double), hipMemcpyDeviceToHost, stream),
                       "hipMemcpy(sum)");
                 check(hipStreamSynchronize(stream), "hipStreamSynchronize");
                 sumOfIncrements = *hSum;
                 sumFromFullEvaluation();
+                if (gradCopied) hostGradFresh = true;
             }
             sumKnown = true;
         }
@@ -441,7 +473,16 @@ struct Engine {
         if (!result) throw Error(MDS_HIP_ERR_ARGUMENT, "null gradient pointer");
         if (length != (size_t)N * P)
             throw Error(MDS_HIP_ERR_ARGUMENT, "getLogLikelihoodGradient: length must be N*P");
-        if (!gradKnown) evaluate(true);
+        gradRequested = true;
+        if (stateEval == StateEval::SumOnly) predictGradient = true;
+        if (hostGradFresh) {  // read back with the sum
+            std::memcpy(result, hGrad, length * sizeof(double));
+            return;
+        }
+        if (!gradKnown) {
+            evaluate(true);
+            if (stateEval == StateEval::None) stateEval = StateEval::Fused;
+        }
         const bool fillSum = !sumKnown;
         check(hipMemcpyAsync(hGrad, dGrad, length * sizeof(double), hipMemcpyDeviceToHost, stream),
               "hipMemcpy(gradient)");
@@ -450,6 +491,7 @@ struct Engine {
                   "hipMemcpy(sum)");
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         std::memcpy(result, hGrad, length * sizeof(double));
+        hostGradFresh = true;
         if (fillSum) {
             sumOfIncrements = *hSum;
             sumKnown = true;

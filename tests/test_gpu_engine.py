@@ -169,6 +169,43 @@ def test_caching_semantics(oracle):
     assert e.getInternalDimension() == p
 
 
+def test_sum_then_gradient_single_pass():
+    """Callers asking for the sum and then the gradient of each state get
+    one fused pass per state once the pattern has been seen, with results
+    bitwise equal to separate evaluations; a switch to sums alone costs one
+    unused gradient and then likelihood passes only; repeated gradient calls
+    of one state launch nothing."""
+    n, p = 300, 2
+    delta, x = _data(n, p, "sym5")
+    xs = [x + 0.05 * k for k in range(6)]
+    ref = []
+    for xk in xs:  # separate engines: a likelihood pass, then a fused pass
+        e, s, g = _gpu(delta, xk, 2.0, True)
+        ref.append((s, g))
+        e.close()
+    e = MultiDimensionalScaling(p, n, _flags(True), 0)
+    e.setPairwiseData(delta)
+    e.setParameters([2.0])
+    f0, _ = e.counters()
+    for k, xk in enumerate(xs):
+        e.updateLocations(-1, xk)
+        s = e.getSumOfIncrements()
+        g = e.getLogLikelihoodGradient().reshape(n, p)
+        assert s == ref[k][0]
+        np.testing.assert_array_equal(g, ref[k][1])
+        np.testing.assert_array_equal(e.getLogLikelihoodGradient().reshape(n, p), g)
+    f1, _ = e.counters()
+    assert f1 - f0 == len(xs) + 1  # two passes for the first state, one for each after
+    for k, xk in enumerate(xs):  # sums alone: the first one still runs the fused pass
+        e.updateLocations(-1, xk)
+        assert e.getSumOfIncrements() == ref[k][0]
+    f2, _ = e.counters()
+    assert f2 - f1 == len(xs)
+    e.updateLocations(-1, xs[0])  # and a gradient afterwards is computed, not stale
+    np.testing.assert_array_equal(e.getLogLikelihoodGradient().reshape(n, p), ref[0][1])
+    e.close()
+
+
 def test_argument_errors():
     delta, x = _data(40, 2, "none")
     e = MultiDimensionalScaling(2, 40, _flags(True), 0)
