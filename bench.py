@@ -388,40 +388,55 @@ def _bench_beast(steps=2000, warmup=20):
 
 
 def _bench_reference_loop(iterations=100, n=1000, p=2, tau=1.0, no_cpu=False, cpu_budget=10.0):
-    """The reference benchmark's loop (src/benchmark.cpp:248-301) on the
+    """The reference benchmark's loop (src/benchmark.cpp:248-307) on the
     reference's own CPU-sized case: N = 1000, P = 2, FP64, euclid10
-    observations without noise or missing pairs, tau = 1, truncation; every
-    iteration draws fresh locations (src/benchmark.cpp:257), uploads them and
-    reads back the sum of increments and the gradient through the API.
-    Reported: that API rate, the device-resident rate of the same evaluation,
-    and the CPU port (serial and all-core SIMD) on the first iterations'
-    locations, with the GPU's relative error against it."""
+    observations without noise or missing pairs, tau = 1, truncation.  Every
+    iteration stores the state, uploads fresh locations (:257), reads the
+    sum of increments, restores the stored state with probability 0.75 or
+    accepts the move (:278-283), and reads the gradient of the state it
+    kept (:296) -- all through the API.  Reported: that rate, the
+    device-resident rate of one fused evaluation, and the CPU port
+    (serial, and all-core SIMD) doing the same work -- a likelihood pass at
+    the moved locations and a gradient pass at the kept ones -- on the first
+    iterations, with the GPU's relative error against the serial one."""
     from massivemds_amd import datagen
     from massivemds_amd.engine import Flags, MultiDimensionalScaling
 
     delta = datagen.euclid10(n)
-    xs = [datagen.locations(n, p, key=1 + k) for k in range(iterations)]
+    xs = [datagen.locations(n, p, key=1 + k) for k in range(iterations + 1)]
+    restore = datagen.uniforms(666, 6, iterations) < 0.75
     e = MultiDimensionalScaling(p, n, Flags.OPENCL | Flags.LEFT_TRUNCATION, 0)
     e.setPairwiseData(delta)
     e.setParameters([tau])
-    sums, grads = [], []
     for k in range(3):  # warm-up
         e.updateLocations(-1, xs[k])
         e.getSumOfIncrements()
         e.getLogLikelihoodGradient()
+    e.updateLocations(-1, xs[0])
+    e.getSumOfIncrements()
+    e.getLogLikelihoodGradient()
+    kept, sums, grads, kept_at = 0, [], [], []
     t0 = time.perf_counter()
     for k in range(iterations):
-        e.updateLocations(-1, xs[k])
+        e.storeState()
+        e.updateLocations(-1, xs[k + 1])
         sums.append(e.getSumOfIncrements())
+        if restore[k]:
+            e.restoreState()
+        else:
+            e.acceptState()
+            kept = k + 1
         grads.append(e.getLogLikelihoodGradient())
+        kept_at.append(kept)
     api_s = time.perf_counter() - t0
     e.close()
     resident = _run_gpu(n, p, tau, delta, xs[0], False, 200, 10)
     out = {
         "workload": "mds_reference_loop_n1000_p2_trunc_tau1",
         "N": n, "P": p, "precision": tau, "iterations": iterations,
+        "restored": int(restore.sum()),
         "api_evals_per_s": iterations / api_s,
-        "api_us_per_eval": api_s / iterations * 1e6,
+        "api_us_per_iteration": api_s / iterations * 1e6,
         "resident_evals_per_s": 1e3 / resident["wall_ms_per_step"],
         "tile_kernel_us": resident["tile_ms"] * 1e3,
         "data": "synthetic (euclid10 distances, no noise, no missing; fresh N(0, I_2) locations per iteration)",
@@ -433,23 +448,24 @@ def _bench_reference_loop(iterations=100, n=1000, p=2, tau=1.0, no_cpu=False, cp
         ser, allc, errs_s, errs_g = [], [], [], []
         k = 0
         while k < iterations and (sum(ser) + sum(allc) < cpu_budget or k < 2):
+            xm, xk = xs[k + 1], xs[kept_at[k]]
             t0 = time.perf_counter()
-            s_ref = oracle.sum_of_increments(delta, xs[k], tau, True)
-            g_ref = oracle.gradient(delta, xs[k], tau, True)
+            s_ref = oracle.sum_of_increments(delta, xm, tau, True)
+            g_ref = oracle.gradient(delta, xk, tau, True)
             ser.append(time.perf_counter() - t0)
             t0 = time.perf_counter()
-            oracle.simd_sum_of_increments(delta, xs[k], tau, True, threads=threads)
-            oracle.simd_gradient(delta, xs[k], tau, True, threads=threads)
+            oracle.simd_sum_of_increments(delta, xm, tau, True, threads=threads)
+            oracle.simd_gradient(delta, xk, tau, True, threads=threads)
             allc.append(time.perf_counter() - t0)
             errs_s.append(abs(sums[k] - s_ref) / abs(s_ref))
             errs_g.append(float(np.max(np.abs(grads[k].reshape(n, p) - g_ref.reshape(n, p)))
                                 / np.max(np.abs(g_ref))))
             k += 1
-        out["cpu_baseline"] = {"value": 1.0 / float(np.median(allc)), "unit": "evals/s", "cores": threads,
+        out["cpu_baseline"] = {"value": 1.0 / float(np.median(allc)), "unit": "iterations/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"the first {k} iterations' evaluations (likelihood pass + gradient "
-                                         f"pass), median; SIMD + OpenMP, {threads} threads"}
-        out["cpu_serial_1core"] = {"value": 1.0 / float(np.median(ser)), "unit": "evals/s", "cores": 1,
+                               "sample": f"the first {k} iterations (likelihood pass at the moved locations + "
+                                         f"gradient pass at the kept ones), median; SIMD + OpenMP, {threads} threads"}
+        out["cpu_serial_1core"] = {"value": 1.0 / float(np.median(ser)), "unit": "iterations/s", "cores": 1,
                                    "sample": f"the first {k} iterations, median; the serial restatement"}
         out["cpu_host"] = _cpu_host()
         out["gpu_vs_cpu"] = {"api_speedup_vs_allcores": out["api_evals_per_s"] * float(np.median(allc)),

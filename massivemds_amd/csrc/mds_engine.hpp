@@ -68,21 +68,29 @@ struct Engine {
     bool devSumFresh = false;  // dSum matches the current state
     bool gradKnown = false;    // dGrad matches the current state
 
-    // Sum-then-gradient prediction.  The reference's callers mostly ask for
-    // the gradient right after the sum of the same state (the benchmark with
-    // --gradient, src/benchmark.cpp:262-270; the R Potential; BEAST's HMC),
+    // Sum-then-gradient prediction.  Callers often ask for the gradient
+    // right after the sum of the same state (the R Potential, BEAST's HMC),
     // which as two API calls would run a likelihood pass and then the fused
-    // pass.  Once a likelihood-only evaluation has been followed by a
-    // gradient request, getSumOfIncrements runs the fused kernels instead and
-    // reads the gradient back with the sum; a fused evaluation whose gradient
-    // then goes unused turns the prediction off again.  Both kernels compute
-    // the sum in the same operations and order, so the sum is bitwise the
-    // same either way; only the work per call changes.
+    // pass.  The engine keeps a running rate of gradient requests per
+    // evaluated state (weight 1/8 per state); above kPredictRate the sum runs
+    // the fused kernels and reads the gradient back with it.  The fused pass
+    // costs ~1.3 likelihood passes, so predicting pays once more than ~0.3
+    // of the states have their gradient asked for.  Both kernels compute the
+    // sum in the same operations and order: the sum is bitwise the same
+    // either way, only the work per call changes.
     enum class StateEval { None, SumOnly, Fused };
-    bool predictGradient = false;
+    static constexpr double kPredictRate = 0.3;
+    double gradRequestRate = 0.0;
     StateEval stateEval = StateEval::None;  // how the current state's sum was evaluated
     bool gradRequested = false;             // the current state's gradient was asked for
     bool hostGradFresh = false;             // hGrad holds the current state's gradient
+    // The gradient of the state storeState saved, when it was on the host
+    // then: restoreState brings it back with the locations and precision, so
+    // a gradient request after a rejected move runs no pass (the values are
+    // those a recomputation gives).  New observations, a new precision or
+    // makeDirty drop it.
+    double* hGradStored = nullptr;  // pinned
+    bool storedGradFresh = false;
 
     // single-row updates: the known sum of the state before row pendingRow
     // moved (its old location in dOldRow) is baseSum
@@ -147,6 +155,8 @@ struct Engine {
             check(hipMalloc((void**)&dSum, sizeof(double)), "hipMalloc(sum)");
             check(hipHostMalloc((void**)&hGrad, (size_t)N * P * sizeof(double), hipHostMallocDefault),
                   "hipHostMalloc(gradient)");
+            check(hipHostMalloc((void**)&hGradStored, (size_t)N * P * sizeof(double), hipHostMallocDefault),
+                  "hipHostMalloc(stored gradient)");
             check(hipHostMalloc((void**)&hSum, sizeof(double), hipHostMallocDefault), "hipHostMalloc(sum)");
             check(hipHostMalloc(&hX, (size_t)npad * kdim * realSize, hipHostMallocDefault),
                   "hipHostMalloc(locations)");
@@ -183,6 +193,7 @@ struct Engine {
         (void)hipFree(dGrad);
         (void)hipFree(dSum);
         (void)hipHostFree(hGrad);
+        (void)hipHostFree(hGradStored);
         (void)hipHostFree(hSum);
         (void)hipHostFree(hX);
         (void)hipFree(dOldRow);
@@ -198,13 +209,14 @@ struct Engine {
         dRowPartial = hRowPartial = nullptr;
         dDelta = dX[0] = dX[1] = nullptr;
         dTiles = nullptr;
-        dTileSums = dPart = dGrad = dSum = hGrad = hSum = nullptr;
+        dTileSums = dPart = dGrad = dSum = hGrad = hGradStored = hSum = nullptr;
         hX = nullptr;
         stream = nullptr;
     }
 
     void invalidate_device_results() {
-        if (stateEval == StateEval::Fused && !gradRequested) predictGradient = false;
+        if (stateEval != StateEval::None)
+            gradRequestRate += 0.125 * ((gradRequested ? 1.0 : 0.0) - gradRequestRate);
         stateEval = StateEval::None;
         gradRequested = false;
         hostGradFresh = false;
@@ -334,6 +346,7 @@ struct Engine {
         dataSet = true;
         sumFromCurrentData = false;  // the cached sum stays (:257-260) but is no base for row updates
         pendingRow = -1;
+        storedGradFresh = false;
         invalidate_device_results();
     }
 
@@ -349,17 +362,21 @@ struct Engine {
             sumKnown = false;
             pendingRow = -1;  // the base sum was for another precision
         }
+        storedGradFresh = false;
         invalidate_device_results();
     }
 
     void makeDirty() {  // :276-279
         sumKnown = false;
         pendingRow = -1;
+        storedGradFresh = false;
         invalidate_device_results();
     }
 
     void storeState() {  // :205-217
         storedSumOfIncrements = sumOfIncrements;
+        storedGradFresh = hostGradFresh;
+        if (hostGradFresh) std::memcpy(hGradStored, hGrad, (size_t)N * P * sizeof(double));
         check(hipMemcpyAsync(dX[1 - cur], dX[cur], (size_t)npad * kdim * realSize,
                              hipMemcpyDeviceToDevice, stream),
               "hipMemcpy(store)");
@@ -373,6 +390,11 @@ struct Engine {
         precision = storedPrecision;
         cur = 1 - cur;
         invalidate_device_results();
+        if (storedGradFresh) {
+            std::swap(hGrad, hGradStored);
+            hostGradFresh = true;
+        }
+        storedGradFresh = false;
     }
 
     hipEvent_t take_event() {
@@ -448,6 +470,7 @@ struct Engine {
             } else {
                 bool gradCopied = false;
                 if (!devSumFresh) {
+                    const bool predictGradient = gradRequestRate > kPredictRate;
                     evaluate(predictGradient);
                     stateEval = predictGradient ? StateEval::Fused : StateEval::SumOnly;
                     if (predictGradient) {
@@ -474,8 +497,7 @@ struct Engine {
         if (length != (size_t)N * P)
             throw Error(MDS_HIP_ERR_ARGUMENT, "getLogLikelihoodGradient: length must be N*P");
         gradRequested = true;
-        if (stateEval == StateEval::SumOnly) predictGradient = true;
-        if (hostGradFresh) {  // read back with the sum
+        if (hostGradFresh) {  // read back with the sum, or restored with the state
             std::memcpy(result, hGrad, length * sizeof(double));
             return;
         }

@@ -169,15 +169,18 @@ def test_caching_semantics(oracle):
     assert e.getInternalDimension() == p
 
 
-def test_sum_then_gradient_single_pass():
+def test_gradient_reuse_and_prediction():
     """Callers asking for the sum and then the gradient of each state get
-    one fused pass per state once the pattern has been seen, with results
-    bitwise equal to separate evaluations; a switch to sums alone costs one
-    unused gradient and then likelihood passes only; repeated gradient calls
-    of one state launch nothing."""
+    one fused pass per state once more than 0.3 of the states have had their
+    gradient asked for (the fourth state here), with results bitwise equal
+    to separate evaluations; repeated gradient calls of one state launch
+    nothing; after storeState / move / restoreState the restored state's
+    gradient comes back without a pass (the reference benchmark's loop,
+    src/benchmark.cpp:254-307); sums alone then cost one pass each and a
+    later gradient is computed, not stale."""
     n, p = 300, 2
     delta, x = _data(n, p, "sym5")
-    xs = [x + 0.05 * k for k in range(6)]
+    xs = [x + 0.05 * k for k in range(8)]
     ref = []
     for xk in xs:  # separate engines: a likelihood pass, then a fused pass
         e, s, g = _gpu(delta, xk, 2.0, True)
@@ -195,12 +198,31 @@ def test_sum_then_gradient_single_pass():
         np.testing.assert_array_equal(g, ref[k][1])
         np.testing.assert_array_equal(e.getLogLikelihoodGradient().reshape(n, p), g)
     f1, _ = e.counters()
-    assert f1 - f0 == len(xs) + 1  # two passes for the first state, one for each after
-    for k, xk in enumerate(xs):  # sums alone: the first one still runs the fused pass
+    assert f1 - f0 == len(xs) + 3  # two passes for each of the first three states
+    e.updateLocations(-1, xs[0])
+    np.testing.assert_array_equal(e.getLogLikelihoodGradient().reshape(n, p), ref[0][1])
+    f1, _ = e.counters()
+    for k in range(1, len(xs)):  # store, move, sum, reject, gradient of the restored state
+        e.storeState()
+        e.updateLocations(-1, xs[k])
+        assert e.getSumOfIncrements() == ref[k][0]
+        e.restoreState()
+        assert e.getSumOfIncrements() == ref[0][0]
+        np.testing.assert_array_equal(e.getLogLikelihoodGradient().reshape(n, p), ref[0][1])
+    f2, _ = e.counters()
+    assert f2 - f1 == len(xs) - 1  # the moved states' sums only
+    e.storeState()  # a new precision drops the stored gradient
+    e.updateLocations(-1, xs[1])
+    e.setParameters([2.0])
+    e.restoreState()
+    np.testing.assert_array_equal(e.getLogLikelihoodGradient().reshape(n, p), ref[0][1])
+    f3, _ = e.counters()
+    assert f3 - f2 == 1
+    for k, xk in enumerate(xs):  # sums alone
         e.updateLocations(-1, xk)
         assert e.getSumOfIncrements() == ref[k][0]
-    f2, _ = e.counters()
-    assert f2 - f1 == len(xs)
+    f4, _ = e.counters()
+    assert f4 - f3 == len(xs)
     e.updateLocations(-1, xs[0])  # and a gradient afterwards is computed, not stale
     np.testing.assert_array_equal(e.getLogLikelihoodGradient().reshape(n, p), ref[0][1])
     e.close()
