@@ -89,8 +89,20 @@ struct Engine {
     // a gradient request after a rejected move runs no pass (the values are
     // those a recomputation gives).  New observations, a new precision or
     // makeDirty drop it.
+    // Copy on write: storeState only marks the current buffer as shared with
+    // the stored state; the next write of a gradient moves to the other
+    // buffer first.
     double* hGradStored = nullptr;  // pinned
     bool storedGradFresh = false;
+    bool storedSharesCurrent = false;  // the stored gradient is the one in hGrad
+
+    // hGrad is about to be overwritten: keep a stored gradient that shares it
+    void own_host_gradient() {
+        if (storedSharesCurrent) {
+            std::swap(hGrad, hGradStored);
+            storedSharesCurrent = false;
+        }
+    }
 
     // single-row updates: the known sum of the state before row pendingRow
     // moved (its old location in dOldRow) is baseSum
@@ -347,6 +359,7 @@ struct Engine {
         sumFromCurrentData = false;  // the cached sum stays (:257-260) but is no base for row updates
         pendingRow = -1;
         storedGradFresh = false;
+        storedSharesCurrent = false;
         invalidate_device_results();
     }
 
@@ -363,6 +376,7 @@ struct Engine {
             pendingRow = -1;  // the base sum was for another precision
         }
         storedGradFresh = false;
+        storedSharesCurrent = false;
         invalidate_device_results();
     }
 
@@ -370,13 +384,14 @@ struct Engine {
         sumKnown = false;
         pendingRow = -1;
         storedGradFresh = false;
+        storedSharesCurrent = false;
         invalidate_device_results();
     }
 
     void storeState() {  // :205-217
         storedSumOfIncrements = sumOfIncrements;
         storedGradFresh = hostGradFresh;
-        if (hostGradFresh) std::memcpy(hGradStored, hGrad, (size_t)N * P * sizeof(double));
+        storedSharesCurrent = hostGradFresh;
         check(hipMemcpyAsync(dX[1 - cur], dX[cur], (size_t)npad * kdim * realSize,
                              hipMemcpyDeviceToDevice, stream),
               "hipMemcpy(store)");
@@ -391,10 +406,11 @@ struct Engine {
         cur = 1 - cur;
         invalidate_device_results();
         if (storedGradFresh) {
-            std::swap(hGrad, hGradStored);
+            if (!storedSharesCurrent) std::swap(hGrad, hGradStored);
             hostGradFresh = true;
         }
         storedGradFresh = false;
+        storedSharesCurrent = false;
     }
 
     hipEvent_t take_event() {
@@ -474,6 +490,7 @@ struct Engine {
                     evaluate(predictGradient);
                     stateEval = predictGradient ? StateEval::Fused : StateEval::SumOnly;
                     if (predictGradient) {
+                        own_host_gradient();
                         check(hipMemcpyAsync(hGrad, dGrad, (size_t)N * P * sizeof(double), hipMemcpyDeviceToHost,
                                              stream),
                               "hipMemcpy(gradient)");
@@ -506,6 +523,7 @@ struct Engine {
             if (stateEval == StateEval::None) stateEval = StateEval::Fused;
         }
         const bool fillSum = !sumKnown;
+        own_host_gradient();
         check(hipMemcpyAsync(hGrad, dGrad, length * sizeof(double), hipMemcpyDeviceToHost, stream),
               "hipMemcpy(gradient)");
         if (fillSum)
