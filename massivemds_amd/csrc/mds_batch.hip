@@ -224,6 +224,15 @@ struct Batch {
     int cuCount = 0;
     size_t realSize = 8;
     hipStream_t stream = nullptr;
+    // One launch per dimension class runs its classes side by side: the
+    // first on `stream`, the others on side streams forked from it and
+    // joined before the reduce, so one class's tail overlaps the next
+    // class's work (the results do not depend on the order: every tile
+    // writes its own slots).
+    static constexpr int kSideStreams = 3;
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t forkEvent = nullptr;
+    hipEvent_t joinEvent[kSideStreams] = {};
     std::vector<BatchEntry> e;
     double* dTileSums = nullptr;
     double* dSums = nullptr;
@@ -275,6 +284,11 @@ struct Batch {
         }
         try {
             check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+            for (int k = 0; k < kSideStreams; ++k) {
+                check(hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking), "hipStreamCreate");
+                check(hipEventCreateWithFlags(&joinEvent[k], hipEventDisableTiming), "hipEventCreate");
+            }
+            check(hipEventCreateWithFlags(&forkEvent, hipEventDisableTiming), "hipEventCreate");
             for (BatchEntry& b : e) {
                 const size_t xBytes = (size_t)b.nb * kTile * b.kdim * realSize;
                 check(hipMalloc(&b.dX, xBytes), "hipMalloc(locations)");
@@ -321,6 +335,17 @@ struct Batch {
         (void)hipFree(dDesc);
         (void)hipFree(dItems);
         (void)hipFree(dReduce);
+        for (int k = 0; k < kSideStreams; ++k) {
+            if (side[k]) {
+                (void)hipStreamSynchronize(side[k]);
+                (void)hipStreamDestroy(side[k]);
+            }
+            if (joinEvent[k]) (void)hipEventDestroy(joinEvent[k]);
+            side[k] = nullptr;
+            joinEvent[k] = nullptr;
+        }
+        if (forkEvent) (void)hipEventDestroy(forkEvent);
+        forkEvent = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         dTileSums = dSums = hSums = nullptr;
         dDesc = nullptr;
@@ -470,7 +495,7 @@ struct Batch {
     }
 
     template <typename Real>
-    void launchClass(int kdim, const BatchItem* items, int count) {
+    void launchClass(int kdim, const BatchItem* items, int count, hipStream_t stream) {
         const int wpb = cfg.wavesPerBlock;
         const int want = (count + wpb - 1) / wpb;
         const int grid = std::max(1, std::min(want, cuCount * cfg.blocksPerCU));
@@ -492,12 +517,28 @@ struct Batch {
 
     void enqueue() {
         if (planDirty) buildPlan();
-        for (const Launch& l : plan) {
+        int used = 0;  // side streams forked
+        if (plan.size() > 1) check(hipEventRecord(forkEvent, stream), "hipEventRecord(fork)");
+        for (size_t c = 0; c < plan.size(); ++c) {
+            const Launch& l = plan[c];
             if (l.count == 0) continue;
+            hipStream_t s = stream;
+            if (c > 0) {
+                const int k = (int)((c - 1) % kSideStreams);
+                s = side[k];
+                if (k >= used) {
+                    check(hipStreamWaitEvent(s, forkEvent, 0), "hipStreamWaitEvent(fork)");
+                    used = k + 1;
+                }
+            }
             if (fp32)
-                launchClass<float>(l.kdim, dItems + l.first, l.count);
+                launchClass<float>(l.kdim, dItems + l.first, l.count, s);
             else
-                launchClass<double>(l.kdim, dItems + l.first, l.count);
+                launchClass<double>(l.kdim, dItems + l.first, l.count, s);
+        }
+        for (int k = 0; k < used; ++k) {
+            check(hipEventRecord(joinEvent[k], side[k]), "hipEventRecord(join)");
+            check(hipStreamWaitEvent(stream, joinEvent[k], 0), "hipStreamWaitEvent(join)");
         }
         hipLaunchKernelGGL(batch_reduce_kernel, dim3(numReduce), dim3(kBatchReduceWaves * kWave), 0, stream, dDesc,
                            dReduce, dTileSums, dSums);
