@@ -105,21 +105,23 @@ struct Engine {
     }
 
     // single-row updates: the known sum of the state before row pendingRow
-    // moved (its old location in dOldRow) is baseSum
+    // moved is baseSum
     static constexpr int kMaxRowUpdates = 1024;
     bool sumFromCurrentData = false;  // the known sum was computed on the current observations
     int pendingRow = -1;
     double baseSum = 0.0;
     int rowUpdatesInARow = 0;
-    void* dOldRow = nullptr;
     double* dRowPartial = nullptr;
     double* hRowPartial = nullptr;  // pinned
-    // single-row uploads: two pinned staging slots, each reused only after
-    // its previous copy has completed (so the call need not wait for it)
-    void* hRowStage = nullptr;
-    hipEvent_t rowStageDone[2] = {nullptr, nullptr};
-    bool rowStageBusy[2] = {false, false};
-    int rowStageSlot = 0;
+    unsigned int* dRowDone = nullptr;  // the row kernel's completion counter
+    // A single-row updateLocations does not touch the device: the location
+    // array keeps the row's previous value, which the row kernel reads
+    // before its last workgroup writes the new one (no copies per move).
+    // Any other use of the locations first writes the pending row
+    // (flush_row_write), which also ends the O(N) path for it.
+    bool rowWritePending = false;
+    int rowWriteIndex = -1;
+    double rowWriteValues[kMaxDim] = {};  // in the engine's real type, widened
     long long fullEvaluations = 0, rowUpdates = 0;
 
     bool timing = false;
@@ -172,13 +174,11 @@ struct Engine {
             check(hipHostMalloc((void**)&hSum, sizeof(double), hipHostMallocDefault), "hipHostMalloc(sum)");
             check(hipHostMalloc(&hX, (size_t)npad * kdim * realSize, hipHostMallocDefault),
                   "hipHostMalloc(locations)");
-            check(hipMalloc(&dOldRow, kdim * realSize), "hipMalloc(old row)");
             check(hipMalloc((void**)&dRowPartial, row_delta_blocks(N) * sizeof(double)), "hipMalloc(row sums)");
             check(hipHostMalloc((void**)&hRowPartial, row_delta_blocks(N) * sizeof(double), hipHostMallocDefault),
                   "hipHostMalloc(row sums)");
-            check(hipHostMalloc(&hRowStage, 2 * kdim * realSize, hipHostMallocDefault), "hipHostMalloc(row)");
-            for (int k = 0; k < 2; ++k)
-                check(hipEventCreateWithFlags(&rowStageDone[k], hipEventDisableTiming), "hipEventCreate");
+            check(hipMalloc((void**)&dRowDone, sizeof(unsigned int)), "hipMalloc(row counter)");
+            check(hipMemsetAsync(dRowDone, 0, sizeof(unsigned int), stream), "hipMemset(row counter)");
             std::memset(hX, 0, (size_t)npad * kdim * realSize);
             check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         } catch (...) {
@@ -208,16 +208,11 @@ struct Engine {
         (void)hipHostFree(hGradStored);
         (void)hipHostFree(hSum);
         (void)hipHostFree(hX);
-        (void)hipFree(dOldRow);
         (void)hipFree(dRowPartial);
         (void)hipHostFree(hRowPartial);
-        (void)hipHostFree(hRowStage);
-        for (int k = 0; k < 2; ++k)
-            if (rowStageDone[k]) (void)hipEventDestroy(rowStageDone[k]);
-        rowStageDone[0] = rowStageDone[1] = nullptr;
-        hRowStage = nullptr;
+        (void)hipFree(dRowDone);
+        dRowDone = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
-        dOldRow = nullptr;
         dRowPartial = hRowPartial = nullptr;
         dDelta = dX[0] = dX[1] = nullptr;
         dTiles = nullptr;
@@ -253,45 +248,54 @@ struct Engine {
             rows = 1;
             row0 = index;
         }
-        // Stage as [rows][kdim] in the engine's real type, padding with zeros:
-        // a single row in the next free row slot, all rows in hX.
-        const int slot = rowStageSlot;
-        void* stage = hX;
-        if (rows == 1) {
-            if (rowStageBusy[slot]) check(hipEventSynchronize(rowStageDone[slot]), "hipEventSynchronize");
-            stage = (char*)hRowStage + slot * kdim * realSize;
-        }
-        if (fp32) {
-            float* s = (float*)stage;
-            for (size_t r = 0; r < rows; ++r)
-                for (int k = 0; k < kdim; ++k) s[r * kdim + k] = k < P ? (float)loc[r * P + k] : 0.0f;
+        if (index >= 0) {
+            if (rowWritePending && rowWriteIndex != index) flush_row_write();  // another row moved
+            if (sumKnown && sumFromCurrentData) {
+                // a single-row update from the known sum: the device row keeps the base value
+                pendingRow = index;
+                baseSum = sumOfIncrements;
+            } else if (index != pendingRow) {
+                pendingRow = -1;  // several rows moved: full recomputation (:150-153)
+            }
+            for (int k = 0; k < P; ++k) rowWriteValues[k] = fp32 ? (double)(float)loc[k] : loc[k];
+            rowWritePending = true;
+            rowWriteIndex = index;
         } else {
-            double* s = (double*)stage;
-            for (size_t r = 0; r < rows; ++r)
-                for (int k = 0; k < kdim; ++k) s[r * kdim + k] = k < P ? loc[r * P + k] : 0.0;
-        }
-        char* dst = (char*)dX[cur] + row0 * kdim * realSize;
-        if (index >= 0 && sumKnown && sumFromCurrentData) {
-            // start a single-row update from the known sum: keep the old row
-            check(hipMemcpyAsync(dOldRow, dst, kdim * realSize, hipMemcpyDeviceToDevice, stream),
-                  "hipMemcpy(old row)");
-            pendingRow = index;
-            baseSum = sumOfIncrements;
-        } else if (index < 0 || index != pendingRow) {
-            pendingRow = -1;  // several rows (or all) moved: full recomputation (:150-153)
-        }
-        check(hipMemcpyAsync(dst, stage, rows * kdim * realSize, hipMemcpyHostToDevice, stream),
-              "hipMemcpy(locations)");
-        if (rows == 1) {
-            // the caller's row is already copied; the slot is reused once this copy is done
-            check(hipEventRecord(rowStageDone[slot], stream), "hipEventRecord");
-            rowStageBusy[slot] = true;
-            rowStageSlot = 1 - slot;
-        } else {
+            // Stage as [N][kdim] in the engine's real type, padding with zeros.
+            rowWritePending = false;  // every row is overwritten
+            pendingRow = -1;
+            if (fp32) {
+                float* st = (float*)hX;
+                for (size_t r = 0; r < rows; ++r)
+                    for (int k = 0; k < kdim; ++k) st[r * kdim + k] = k < P ? (float)loc[r * P + k] : 0.0f;
+            } else {
+                double* st = (double*)hX;
+                for (size_t r = 0; r < rows; ++r)
+                    for (int k = 0; k < kdim; ++k) st[r * kdim + k] = k < P ? loc[r * P + k] : 0.0;
+            }
+            check(hipMemcpyAsync((char*)dX[cur] + row0 * kdim * realSize, hX, rows * kdim * realSize,
+                                 hipMemcpyHostToDevice, stream),
+                  "hipMemcpy(locations)");
             check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         }
         sumKnown = false;
         invalidate_device_results();
+    }
+
+    // Writes the pending single-row update into the location array; the
+    // O(N) path for it ends (its previous location is gone).
+    void flush_row_write() {
+        if (!rowWritePending) return;
+        RowWrite w;
+        w.x = dX[cur];
+        for (int k = 0; k < P; ++k) w.values[k] = rowWriteValues[k];
+        w.row = rowWriteIndex;
+        w.kdim = kdim;
+        w.p = P;
+        w.fp32 = fp32;
+        check(launch_row_write(w, stream), "row write kernel");
+        rowWritePending = false;
+        pendingRow = -1;
     }
 
     // The raw sum after the pending single-row update: base + half the
@@ -300,11 +304,13 @@ struct Engine {
         RowDeltaLaunch a;
         a.delta = dDelta;
         a.x = dX[cur];
-        a.xOld = dOldRow;
+        for (int k = 0; k < P; ++k) a.xNew[k] = rowWriteValues[k];
         a.row = pendingRow;
         a.n = N;
         a.nb = nb;
         a.kdim = kdim;
+        a.p = P;
+        a.done = dRowDone;
         a.planeStride = (int64_t)numTiles * kTileElems;
         a.tau = precision;
         a.truncated = truncated;
@@ -312,6 +318,7 @@ struct Engine {
         a.fp32 = fp32;
         a.partial = dRowPartial;
         check(launch_row_delta(a, stream), "row update kernel");
+        rowWritePending = false;  // the kernel's last workgroup writes the row
         const int blocks = row_delta_blocks(N);
         check(hipMemcpyAsync(hRowPartial, dRowPartial, blocks * sizeof(double), hipMemcpyDeviceToHost, stream),
               "hipMemcpy(row sums)");
@@ -389,6 +396,7 @@ struct Engine {
     }
 
     void storeState() {  // :205-217
+        flush_row_write();
         storedSumOfIncrements = sumOfIncrements;
         storedGradFresh = hostGradFresh;
         storedSharesCurrent = hostGradFresh;
@@ -399,6 +407,7 @@ struct Engine {
     }
 
     void restoreState() {  // :234-255
+        flush_row_write();
         sumOfIncrements = storedSumOfIncrements;
         sumKnown = true;
         pendingRow = -1;
@@ -430,6 +439,7 @@ struct Engine {
     void evaluate(bool gradient, const double* devTau = nullptr) {
         if (!dataSet) throw Error(MDS_HIP_ERR_STATE, "pairwise data not set");
         if (!precisionSet) throw Error(MDS_HIP_ERR_STATE, "precision not set");
+        flush_row_write();
         hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
         if (timing) {
             e0 = take_event();
@@ -480,7 +490,8 @@ struct Engine {
 
     double getSumOfIncrements() {  // :193-203
         if (!sumKnown) {
-            if (pendingRow >= 0 && !devSumFresh && rowUpdatesInARow < kMaxRowUpdates) {
+            if (pendingRow >= 0 && !devSumFresh && rowUpdatesInARow < kMaxRowUpdates && rowWritePending &&
+                rowWriteIndex == pendingRow) {
                 applyRowUpdate();
                 pendingRow = -1;
             } else {

@@ -64,19 +64,31 @@ constexpr int kRowThreads = 256;
 
 struct RowDeltaLaunch {
     const void* delta;       // packed tiles as in TileLaunch
-    const void* x;           // Real [nb * 64][kdim]: row `row` already holds the new location
-    const void* xOld;        // Real [kdim]: the row's previous location
+    void* x;                 // Real [nb * 64][kdim]: row `row` still holds the previous location;
+                             // the last workgroup to finish writes xNew there
+    double xNew[kMaxDim];    // the row's new location (the engine's real type, widened), P values
     int row;
     int n;
     int nb;
     int kdim;
+    int p;
     int64_t planeStride;
     double tau;
     bool truncated;
     bool symmetric;
     bool fp32;               // Real = float (else double)
     double* partial;         // [blocks(n)]
+    unsigned int* done;      // workgroups finished; 0 before the launch, reset by the last one
 };
+
+// Writes one location row (the engine's real type; padding dimensions stay 0).
+struct RowWrite {
+    void* x;
+    double values[kMaxDim];
+    int row, kdim, p;
+    bool fp32;
+};
+hipError_t launch_row_write(const RowWrite& w, hipStream_t s);
 
 inline int row_delta_blocks(int n) { return (n + kRowThreads - 1) / kRowThreads; }
 

@@ -4,8 +4,9 @@
 // built for the packed tile layout.
 //
 // Thread j (one per location) finds delta_ij and delta_ji in the packed
-// upper-triangle tiles, computes the pair's distance with the row's old and
-// new location, and returns the change of the pair's terms (the same terms as
+// upper-triangle tiles, computes the pair's distance with the row's old
+// location (still in the location array) and its new one (a kernel
+// argument), and returns the change of the pair's terms (the same terms as
 // the tile kernel: 0.5 tau r^2 + log Phi(d sqrt tau) with truncation, r^2
 // without), in FP64 for both engine precisions.  Each block sums its threads
 // in a fixed order (DPP wavefront sums, then the four wavefronts in order).
@@ -44,11 +45,16 @@ __device__ __forceinline__ double pair_term(double obs, double d, double halfTau
     return fma(halfTau * r, r, trunc64_table(d, sT, coeffs::kLnPhiTab).logPhi);
 }
 
+// The row's previous location is read from x (every workgroup reads it
+// before it signals completion); the last workgroup to finish -- after all
+// the others' reads -- writes the new location into x and resets the
+// counter for the next launch.  The counter orders the write only; the
+// change itself is summed per workgroup in a fixed order and the partials
+// are added on the host in order.
 template <typename Real, int P>
 __global__ __launch_bounds__(kRowThreads) void row_delta_kernel(RowDeltaLaunch a) {
     const Real* delta = (const Real*)a.delta;
-    const Real* x = (const Real*)a.x;
-    const Real* xOld = (const Real*)a.xOld;
+    Real* x = (Real*)a.x;
     const int j = blockIdx.x * kRowThreads + threadIdx.x;
     const double halfTau = 0.5 * a.tau;
     const double sT = coeffs::kLnPhiTabStep * sqrt(a.tau);
@@ -60,8 +66,8 @@ __global__ __launch_bounds__(kRowThreads) void row_delta_kernel(RowDeltaLaunch a
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const double xj = (double)x[(int64_t)j * P + p];
-            const double dn = (double)x[(int64_t)a.row * P + p] - xj;
-            const double dold = (double)xOld[p] - xj;
+            const double dn = (p < a.p ? a.xNew[p] : 0.0) - xj;
+            const double dold = (double)x[(int64_t)a.row * P + p] - xj;
             sNew = fma(dn, dn, sNew);
             sOld = fma(dold, dold, sOld);
         }
@@ -74,12 +80,34 @@ __global__ __launch_bounds__(kRowThreads) void row_delta_kernel(RowDeltaLaunch a
     __shared__ double waveSums[kRowThreads / kWave];
     const double w = wave_sum_to_last(change);
     if (threadIdx.x % kWave == kWave - 1) waveSums[threadIdx.x / kWave] = w;
-    __syncthreads();
+    __syncthreads();  // also: every thread of the workgroup has read the old row
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int k = 0; k < kRowThreads / kWave; ++k) s += waveSums[k];
         a.partial[blockIdx.x] = s;
+        __threadfence();
+        const unsigned int prev = atomicAdd(a.done, 1u);
+        if (prev == gridDim.x - 1) {
+            __threadfence();
+            for (int p = 0; p < a.p; ++p) x[(int64_t)a.row * P + p] = (Real)a.xNew[p];
+            atomicExch(a.done, 0u);
+        }
     }
+}
+
+template <typename Real>
+__global__ void row_write_kernel(RowWrite w) {
+    const int p = threadIdx.x;
+    if (p < w.p) ((Real*)w.x)[(int64_t)w.row * w.kdim + p] = (Real)w.values[p];
+}
+
+hipError_t launch_row_write(const RowWrite& w, hipStream_t s) {
+    if (w.p < 1 || w.p > kMaxDim || w.p > w.kdim || w.row < 0) return hipErrorInvalidValue;
+    if (w.fp32)
+        hipLaunchKernelGGL(row_write_kernel<float>, dim3(1), dim3(kMaxDim), 0, s, w);
+    else
+        hipLaunchKernelGGL(row_write_kernel<double>, dim3(1), dim3(kMaxDim), 0, s, w);
+    return hipGetLastError();
 }
 
 template <typename Real>
@@ -107,7 +135,8 @@ static hipError_t row_delta_dim(const RowDeltaLaunch& a, hipStream_t s) {
 }
 
 hipError_t launch_row_delta(const RowDeltaLaunch& a, hipStream_t s) {
-    if (a.row < 0 || a.row >= a.n || a.nb * kTile < a.n) return hipErrorInvalidValue;
+    if (a.row < 0 || a.row >= a.n || a.nb * kTile < a.n || a.p < 1 || a.p > a.kdim || !a.done)
+        return hipErrorInvalidValue;
     return a.fp32 ? row_delta_dim<float>(a, s) : row_delta_dim<double>(a, s);
 }
 

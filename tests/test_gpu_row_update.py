@@ -130,3 +130,69 @@ def test_many_updates_stay_exact_and_refresh():
     ref = _fresh_sum(n, p, True, False, delta, x)
     assert abs(s - ref) <= 1e-12 * abs(ref)
     e.close()
+
+
+def _fresh(n, p, delta, x, fp32=False):
+    e = _engine(n, p, True, fp32, delta, x)
+    s = e.getSumOfIncrements()
+    g = e.getLogLikelihoodGradient().reshape(n, p)
+    e.close()
+    return s, g
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_deferred_row_write_sequences(fp32):
+    """A single-row updateLocations leaves the device row at its previous
+    value until the row kernel writes the new one; every other use of the
+    locations writes it first.  Sequences that take those other routes
+    give the values of a fresh engine at the same locations."""
+    n, p = 257, 3
+    delta = datagen.euclid10(n, 51, 0.1, 0.05)
+    x = datagen.locations(n, p, 52)
+    tol = 2e-6 if fp32 else 1e-12
+
+    def close(e, xs):
+        s_ref, g_ref = _fresh(n, p, delta, xs, fp32)
+        s, g = e.getSumOfIncrements(), e.getLogLikelihoodGradient().reshape(n, p)
+        assert abs(s - s_ref) <= tol * abs(s_ref), (s, s_ref)
+        assert np.max(np.abs(g - g_ref)) <= (10 * tol if fp32 else tol) * np.max(np.abs(g_ref))
+
+    e = _engine(n, p, True, fp32, delta, x)
+    e.getSumOfIncrements()
+    # move, then store before the sum: the store writes the row, the sum is a full pass
+    x[5] += 0.3
+    e.updateLocations(5, x[5])
+    e.storeState()
+    close(e, x)
+    # move a row, then all rows: the pending row is superseded
+    x2 = x + 0.01
+    x2[9] -= 0.4
+    e.updateLocations(9, x2[9] + 5.0)
+    e.updateLocations(-1, x2)
+    close(e, x2)
+    # move one row, then another, then the gradient alone
+    x3 = x2.copy()
+    x3[1] += 0.2
+    x3[200] -= 0.2
+    e.updateLocations(1, x3[1])
+    e.updateLocations(200, x3[200])
+    g = e.getLogLikelihoodGradient().reshape(n, p)
+    g_ref = _fresh(n, p, delta, x3, fp32)[1]
+    assert np.max(np.abs(g - g_ref)) <= (10 * tol if fp32 else tol) * np.max(np.abs(g_ref))
+    close(e, x3)
+    # store, move without asking for the sum, restore: the stored state comes back whole
+    e.storeState()
+    s_stored = e.getSumOfIncrements()
+    x4 = x3.copy()
+    x4[77] += 1.0
+    e.updateLocations(77, x4[77])
+    e.restoreState()
+    assert e.getSumOfIncrements() == s_stored
+    close(e, x3)
+    # the same row three times, then the sum (O(N) from the first base) and the gradient
+    e.getSumOfIncrements()
+    for step in range(3):
+        x3[42] += 0.1 * (step + 1)
+        e.updateLocations(42, x3[42])
+    close(e, x3)
+    e.close()
