@@ -138,6 +138,36 @@ __global__ __launch_bounds__(kEnergyThreads) void energy_kernel(const Real* __re
     }
 }
 
+// The trace's rows (kTraceRows of them, indices in rows), widened to double:
+// out[k * p + d].
+template <typename Real>
+__global__ void gather_rows_kernel(const Real* __restrict__ x, int kdim, int p, TraceRows rows,
+                                   double* __restrict__ out) {
+    const int t = threadIdx.x;
+    if (t < kTraceRows * p) out[t] = (double)x[(int64_t)rows.r[t / p] * kdim + t % p];
+}
+
+// X^T X: workgroup (a, b) sums x_ia x_ib over i in a fixed order (strided
+// thread sums, then a fixed tree), so entries (a, b) and (b, a) agree bitwise.
+constexpr int kGramThreads = 256;
+
+template <typename Real>
+__global__ __launch_bounds__(kGramThreads) void gram_kernel(const Real* __restrict__ x, int n, int kdim, int p,
+                                                            double* __restrict__ out) {
+    __shared__ double red[kGramThreads];
+    const int a = blockIdx.x / p, b = blockIdx.x % p;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kGramThreads)
+        acc += (double)x[(int64_t)i * kdim + a] * (double)x[(int64_t)i * kdim + b];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kGramThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[a * p + b] = red[0];
+}
+
 // ------------------------------------------------------------------ host
 
 // Marsaglia & Tsang (2000), "A simple method for generating gamma variables".
@@ -258,6 +288,14 @@ Hmc::Hmc(Engine& engine, const mds_hip_hmc_options& options) : e(engine), opt(op
     check(hipHostMalloc((void**)&hParams, kHmcParamCount * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
     check(hipHostMalloc((void**)&hScalars, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
     check(hipHostMalloc(&hX, xBytes, hipHostMallocDefault), "hipHostMalloc");
+    check(hipMalloc((void**)&dRows, kTraceRows * kMaxDim * sizeof(double)), "hipMalloc(trace rows)");
+    check(hipMalloc((void**)&dGram, kMaxDim * kMaxDim * sizeof(double)), "hipMalloc(gram)");
+    check(hipHostMalloc((void**)&hRows, kTraceRows * kMaxDim * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+    check(hipHostMalloc((void**)&hGram, kMaxDim * kMaxDim * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+    {
+        const int r[kTraceRows] = {0, 1 % n, n / 4, n / 3, n / 2, 2 * n / 3, 3 * n / 4, n - 1};
+        for (int k = 0; k < kTraceRows; ++k) traceRows.r[k] = r[k];
+    }
     std::memset(hParams, 0, kHmcParamCount * sizeof(double));
     tau = e.precision;
     trait.assign((size_t)p * p, 0.0);
@@ -287,6 +325,10 @@ Hmc::~Hmc() {
     (void)hipHostFree(hParams);
     (void)hipHostFree(hScalars);
     (void)hipHostFree(hX);
+    (void)hipFree(dRows);
+    (void)hipFree(dGram);
+    (void)hipHostFree(hRows);
+    (void)hipHostFree(hGram);
 }
 
 void Hmc::uploadParams() {
@@ -379,6 +421,31 @@ double Hmc::logPrior(double quad) const {
     return -0.5 * quad - 0.5 * n * p * std::log(2.0 * M_PI) + 0.5 * n * log_det_spd(trait, p);
 }
 
+// Enqueues the trace rows (record) and X^T X (trait) of the current
+// locations into pinned host memory; the caller synchronises before reading.
+void Hmc::gatherTraceRows() {
+    if (e.fp32)
+        hipLaunchKernelGGL(gather_rows_kernel<float>, dim3(1), dim3(kTraceRows * kMaxDim), 0, e.stream,
+                           (const float*)e.dX[e.cur], e.kdim, p, traceRows, dRows);
+    else
+        hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(1), dim3(kTraceRows * kMaxDim), 0, e.stream,
+                           (const double*)e.dX[e.cur], e.kdim, p, traceRows, dRows);
+    check(hipGetLastError(), "trace rows kernel");
+    check(hipMemcpyAsync(hRows, dRows, kTraceRows * p * sizeof(double), hipMemcpyDeviceToHost, e.stream),
+          "hipMemcpy(trace rows)");
+}
+
+void Hmc::gatherGram() {
+    if (e.fp32)
+        hipLaunchKernelGGL(gram_kernel<float>, dim3(p * p), dim3(kGramThreads), 0, e.stream, (const float*)e.dX[e.cur],
+                           n, e.kdim, p, dGram);
+    else
+        hipLaunchKernelGGL(gram_kernel<double>, dim3(p * p), dim3(kGramThreads), 0, e.stream,
+                           (const double*)e.dX[e.cur], n, e.kdim, p, dGram);
+    check(hipGetLastError(), "gram kernel");
+    check(hipMemcpyAsync(hGram, dGram, p * p * sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy(gram)");
+}
+
 void Hmc::copyLocationsToHost(std::vector<double>& out) {
     const size_t bytes = (size_t)n * e.kdim * e.realSize;
     check(hipMemcpyAsync(hX, e.dX[e.cur], bytes, hipMemcpyDeviceToHost, e.stream), "hipMemcpy(locations)");
@@ -390,25 +457,28 @@ void Hmc::copyLocationsToHost(std::vector<double>& out) {
                                             : ((const double*)hX)[(size_t)i * e.kdim + d];
 }
 
-void Hmc::record(double* row, const std::vector<double>& x, double logPost) const {
+// rows: the trace rows in the order of traceRows (0, 1, n/4, n/3, n/2,
+// 2n/3, 3n/4, n - 1), p values each
+void Hmc::record(double* row, const double* rows, double logPost) const {
     row[0] = logPost;
     row[1] = tau;
     double tr = 0.0;
     for (int d = 0; d < p; ++d) tr += trait[d * p + d];
     row[2] = tr;
-    const int pairs[5][2] = {{0, 1}, {0, n / 2}, {n / 3, 2 * n / 3}, {1, n - 1}, {n / 4, 3 * n / 4}};
+    // distances 0-1, 0-n/2, n/3-2n/3, 1-(n-1), n/4-3n/4; norms of 0, n/2, n-1
+    const int pairs[5][2] = {{0, 1}, {0, 4}, {3, 5}, {1, 7}, {2, 6}};
     for (int k = 0; k < 5; ++k) {
         double s = 0.0;
         for (int d = 0; d < p; ++d) {
-            const double v = x[(size_t)pairs[k][0] * p + d] - x[(size_t)pairs[k][1] * p + d];
+            const double v = rows[pairs[k][0] * p + d] - rows[pairs[k][1] * p + d];
             s += v * v;
         }
         row[3 + k] = std::sqrt(s);
     }
-    const int rows[3] = {0, n / 2, n - 1};
+    const int norms[3] = {0, 4, 7};
     for (int k = 0; k < 3; ++k) {
         double s = 0.0;
-        for (int d = 0; d < p; ++d) s += x[(size_t)rows[k] * p + d] * x[(size_t)rows[k] * p + d];
+        for (int d = 0; d < p; ++d) s += rows[norms[k] * p + d] * rows[norms[k] * p + d];
         row[8 + k] = std::sqrt(s);
     }
 }
@@ -460,8 +530,16 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
         hipLaunchKernelGGL(momentum_kernel, dim3((count / 2 + threads) / threads), dim3(threads), 0, e.stream, dMom,
                            count, (uint64_t)opt.seed, (uint64_t)it);
         check(hipGetLastError(), "momentum kernel");
-        readEnergies();
-        const double K0 = hScalars[0];
+        // the starting kinetic energy is read back with the energies after the
+        // trajectory (no wait here)
+        if (e.fp32)
+            hipLaunchKernelGGL((energy_kernel<float>), dim3(1), dim3(kEnergyThreads), 0, e.stream,
+                               (const float*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, dEnergy);
+        else
+            hipLaunchKernelGGL((energy_kernel<double>), dim3(1), dim3(kEnergyThreads), 0, e.stream,
+                               (const double*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, dEnergy);
+        check(hipGetLastError(), "energy kernel");
+        check(hipMemcpyAsync(hScalars + 3, dEnergy, sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
         check(hipMemcpyAsync(dXSave, e.dX[e.cur], xBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
         check(hipMemcpyAsync(dGradSave, e.dGrad, gBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
         if (mode == kExactCollect)  // snapshots of a rejected trajectory are dropped
@@ -481,6 +559,7 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             ++exactIts;
         }
         readEnergies();
+        const double K0 = hScalars[3];
         const double K1 = hScalars[0];
         const double logLik1 = logLikelihood(hScalars[2], tau);
         const double quad1 = hScalars[1];
@@ -530,6 +609,14 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
                 }
             }
         }
+
+        // the trace rows and X^T X of the kept locations (the precision step
+        // does not move them), read after the next wait
+        const bool traitStep = opt.learnTraitPrecision && it < opt.iterations - 1 &&
+                               ((it + 1) % opt.traitInvWeight == 0);
+        const bool recordIt = it >= opt.burnIn && trace;
+        if (traitStep) gatherGram();
+        if (recordIt) gatherTraceRows();
 
         if (opt.learnPrecision) {
             if (gradMode == kSurrogate) {
@@ -587,26 +674,19 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             }
         }
 
-        const bool traitStep = opt.learnTraitPrecision && it < opt.iterations - 1 &&
-                               ((it + 1) % opt.traitInvWeight == 0);
-        if (traitStep || it >= opt.burnIn) copyLocationsToHost(xHost);
+        if (traitStep || recordIt) check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
         if (traitStep) {
             std::vector<double> m((size_t)p * p, 0.0);  // T0 + X^T X, T0 = I_P
             for (int d = 0; d < p; ++d) m[d * p + d] = 1.0;
-            for (int i = 0; i < n; ++i)
-                for (int a = 0; a < p; ++a)
-                    for (int b = 0; b < p; ++b) m[a * p + b] += xHost[(size_t)i * p + a] * xHost[(size_t)i * p + b];
+            for (int k = 0; k < p * p; ++k) m[k] += hGram[k];
             const double df = (opt.priorDf > 0 ? opt.priorDf : p) + n;
             trait = wishart_draw(rng, df, spd_inverse(m, p), p);
             uploadParams();
-            quad = 0.0;
-            for (int i = 0; i < n; ++i)
-                for (int a = 0; a < p; ++a)
-                    for (int b = 0; b < p; ++b)
-                        quad += xHost[(size_t)i * p + a] * trait[a * p + b] * xHost[(size_t)i * p + b];
+            quad = 0.0;  // sum_i x_i^T T x_i = sum_ab T_ab (X^T X)_ab
+            for (int k = 0; k < p * p; ++k) quad += trait[k] * hGram[k];
             U = -(logLik + logPrior(quad));
         }
-        if (it >= opt.burnIn && trace) record(trace + (size_t)(it - opt.burnIn) * kHmcTraceColumns, xHost, -U);
+        if (recordIt) record(trace + (size_t)(it - opt.burnIn) * kHmcTraceColumns, hRows, -U);
     }
     check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
     const auto tEnd = clock::now();

@@ -16,6 +16,13 @@ constexpr int kHmcTrait = 2;
 constexpr int kHmcParamCount = kHmcTrait + kMaxDim * kMaxDim;
 constexpr int kHmcTraceColumns = MDS_HIP_HMC_TRACE_COLUMNS;
 
+// The locations the trace records (R/Untitled.R's invariant functionals):
+// rows 0, 1, n/4, n/3, n/2, 2n/3, 3n/4, n - 1.
+constexpr int kTraceRows = 8;
+struct TraceRows {
+    int r[kTraceRows];
+};
+
 struct Hmc {
     Engine& e;
     mds_hip_hmc_options opt;
@@ -30,6 +37,11 @@ struct Hmc {
     double* hParams = nullptr;
     double* hScalars = nullptr;
     void* hX = nullptr;
+    double* dRows = nullptr;  // trace rows [kTraceRows][p]
+    double* dGram = nullptr;  // X^T X [p][p]
+    double* hRows = nullptr;  // pinned
+    double* hGram = nullptr;  // pinned
+    TraceRows traceRows{};
     // gradient sources of a leapfrog step
     enum Mode { kExact = 0, kExactCollect = 1, kSurrogate = 2 };
     Surrogate* sur = nullptr;
@@ -49,7 +61,9 @@ struct Hmc {
     double logLikelihood(double rawSum, double t) const;
     double logPrior(double quad) const;
     void copyLocationsToHost(std::vector<double>& out);
-    void record(double* row, const std::vector<double>& x, double logPost) const;
+    void gatherTraceRows();
+    void gatherGram();
+    void record(double* row, const double* rows, double logPost) const;
     void run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats* stats);
 };
 
