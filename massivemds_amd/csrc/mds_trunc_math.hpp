@@ -41,8 +41,7 @@ struct HwSeeds {
     __device__ static int mul24(int a, int b) { return __mul24(a, b); }
 };
 
-// 2^500: d2 * kDistBig exceeds sqrt(d2) for every d2 > 2^-1000, and is 0 at
-// d2 = 0, where the Newton step gives NaN (0 * inf).
+// 2^500: above 1/sqrt(d2) for every d2 > 2^-1000.
 constexpr double kDistBig = 0x1p500;
 
 // d = sqrt(d2) and 1/d.  d2 = 0 gives d = 0 and 1/d = NaN (there the
@@ -50,8 +49,10 @@ constexpr double kDistBig = 0x1p500;
 // masked pairs are zeroed after); a NaN d2 gives NaN for both, d2 = inf gives
 // d = inf.  The seed's relative error e0 (v_rsq_f64: ~1.7e-8) becomes
 // ~1.5 e0^2 after NEWTON = 1 step (~5e-16) and rounding level after 2.
-// The minimum with d2 * 2^500 (IEEE minNum: a NaN operand yields the other)
-// turns the NaN of d2 = 0 into 0 and leaves every d2 > 2^-1000 unchanged.
+// d multiplies d2 by min(1/d, 2^500) (IEEE minNum: a NaN operand yields the
+// other): the NaN that the Newton step gives at d2 = 0 (0 * inf) and at
+// d2 = inf becomes 2^500, so d = 0 and d = inf there, and every
+// d2 > 2^-1000 keeps d = d2 / sqrt(d2) bit for bit.
 template <class S, int NEWTON = 1>
 __host__ __device__ inline void distance64(double d2, double& d, double& invd) {
     double y = S::rsq(d2);
@@ -62,14 +63,19 @@ __host__ __device__ inline void distance64(double d2, double& d, double& invd) {
         y = fma(y, fma(-hy, y, 0.5), y);
     }
     invd = y;
-    d = fmin(d2 * y, d2 * kDistBig);
+    d = d2 * fmin(y, kDistBig);
 }
 
+// 2^64: above 1/sqrt(d2) for every normal FP32 d2.
+constexpr float kDistBig32 = 0x1p64f;
+
+// FP32: the seed alone (~1 ulp).  d2 = 0: 1/d = inf, d = 0 * 2^64 = 0;
+// d2 = inf: 1/d = 0 and d = inf * 0 = NaN, as the plain product gives.
 template <class S>
 __host__ __device__ inline void distance32(float d2, float& d, float& invd) {
     const float y = S::rsq(d2);
     invd = y;
-    d = d2 == 0.0f ? 0.0f : d2 * y;
+    d = d2 * fminf(y, kDistBig32);
 }
 
 template <typename Real>
