@@ -86,6 +86,10 @@ def test_fp64_table_edges():
     d, invd, log_phi, grad = _eval(False, np.array([np.inf, np.nan]), tau)
     assert np.all(log_phi == 0.0) and np.all(grad == 0.0)
     assert np.isnan(d[1]) and np.isnan(invd[1])  # NaN propagates through d (and so the residual)
+    assert d[0] == np.inf  # an infinite distance stays infinite (the residual is -inf)
+    # the guard on 1/d leaves tiny but representable distances exact
+    d, invd, _, _ = _eval(False, np.array([1e-300, 2.0 ** -998]), tau)
+    assert abs(d[0] / 1e-150 - 1) <= 8e-16 and abs(d[1] / 2.0 ** -499 - 1) <= 8e-16
 
 
 def test_fp64_zero_distance():
