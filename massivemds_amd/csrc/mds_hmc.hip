@@ -508,7 +508,6 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
     long long accepted = 0, proposedTau = 0, acceptedTau = 0, evaluations = 1;
     long long surrogateIts = 0, exactIts = 0, surrogateAccepted = 0;
     int windowIts = 0, windowAcc = 0, fallbackLeft = 0, failedWindows = 0;
-    std::vector<double> xHost;
     auto tSampling = clock::now();
     const int threads = 256;
     for (int it = 0; it < opt.iterations; ++it) {
@@ -691,6 +690,7 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
     check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
     const auto tEnd = clock::now();
     if (finalX) {
+        std::vector<double> xHost;
         copyLocationsToHost(xHost);
         std::memcpy(finalX, xHost.data(), xHost.size() * sizeof(double));
     }
