@@ -137,6 +137,54 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batch_tile_kernel(const Batc
 
 constexpr int kBatchReduceWaves = 8;
 
+// Gradient rows of row block K of one dataset: each of the 8 wavefronts adds
+// the column blocks J = w mod 8 for every coordinate at once (the loads of
+// kAhead column blocks of all KD coordinates in flight together), then the
+// wavefront partials are combined per coordinate in wavefront order.  Per
+// coordinate the additions are those of the single-instance reduce, in the
+// same order, so the rows are bitwise the same.
+template <int KD>
+__device__ __forceinline__ void batch_reduce_rows(const BatchDataset& d, int K, double* red) {
+    constexpr int kAhead = 4;
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    double acc[KD];
+#pragma unroll
+    for (int p = 0; p < KD; ++p) acc[p] = 0.0;
+    const int64_t stride = (int64_t)KD * kTile;  // between column blocks J
+    const double* base = d.part + (int64_t)K * d.nb * stride + lane;
+    int J = wave;
+    for (; J + (kAhead - 1) * kBatchReduceWaves < d.nb; J += kAhead * kBatchReduceWaves) {
+        double v[kAhead][KD];
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            const double* slot = base + (J + u * kBatchReduceWaves) * stride;
+#pragma unroll
+            for (int p = 0; p < KD; ++p) v[u][p] = slot[p * kTile];
+        }
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u)
+#pragma unroll
+            for (int p = 0; p < KD; ++p) acc[p] += v[u][p];
+    }
+    for (; J < d.nb; J += kBatchReduceWaves) {
+        const double* slot = base + J * stride;
+#pragma unroll
+        for (int p = 0; p < KD; ++p) acc[p] += slot[p * kTile];
+    }
+    const int row = K * kTile + lane;
+#pragma unroll
+    for (int p = 0; p < KD; ++p) {
+        red[tid] = acc[p];
+        __syncthreads();
+        if (wave == 0) {
+            double v = 0.0;
+            for (int w = 0; w < kBatchReduceWaves; ++w) v += red[w * kWave + lane];
+            if (row < d.n && p < d.p) d.grad[(int64_t)row * d.p + p] = v;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(kBatchReduceWaves * kWave) void batch_reduce_kernel(
     const BatchDataset* __restrict__ ds, const ReduceItem* __restrict__ items,
     const double* __restrict__ tileSums, double* __restrict__ sums) {
@@ -144,10 +192,10 @@ __global__ __launch_bounds__(kBatchReduceWaves * kWave) void batch_reduce_kernel
     const ReduceItem item = items[blockIdx.x];
     const BatchDataset d = ds[item.ds];
     const int tid = threadIdx.x;
-    // loads issued kAhead iterations at a time, added in the plain loop's order
-    constexpr int kAhead = 4;
-    constexpr int kThreads = kBatchReduceWaves * kWave;
     if (item.K < 0) {
+        // loads issued kAhead iterations at a time, added in the plain loop's order
+        constexpr int kAhead = 4;
+        constexpr int kThreads = kBatchReduceWaves * kWave;
         double acc = 0.0;
         const double* ts = tileSums + d.tileSumBase;
         int k = tid;
@@ -168,30 +216,16 @@ __global__ __launch_bounds__(kBatchReduceWaves * kWave) void batch_reduce_kernel
         if (tid == 0) sums[item.ds] = 0.5 * red[0];
         return;
     }
-    const int wave = tid / kWave, lane = tid % kWave;
-    const int K = item.K;
-    const int row = K * kTile + lane;
-    for (int p = 0; p < d.kdim; ++p) {
-        double acc = 0.0;
-        const double* col = d.part + ((int64_t)K * d.nb * d.kdim + p) * kTile + lane;  // + J * kdim * 64
-        const int64_t stride = (int64_t)d.kdim * kTile;
-        int J = wave;
-        for (; J + (kAhead - 1) * kBatchReduceWaves < d.nb; J += kAhead * kBatchReduceWaves) {
-            double v[kAhead];
-#pragma unroll
-            for (int u = 0; u < kAhead; ++u) v[u] = col[(J + u * kBatchReduceWaves) * stride];
-#pragma unroll
-            for (int u = 0; u < kAhead; ++u) acc += v[u];
-        }
-        for (; J < d.nb; J += kBatchReduceWaves) acc += col[J * stride];
-        red[tid] = acc;
-        __syncthreads();
-        if (wave == 0) {
-            double v = 0.0;
-            for (int w = 0; w < kBatchReduceWaves; ++w) v += red[w * kWave + lane];
-            if (row < d.n && p < d.p) d.grad[(int64_t)row * d.p + p] = v;
-        }
-        __syncthreads();
+    switch (d.kdim) {  // uniform over the block
+        case 1: batch_reduce_rows<1>(d, item.K, red); break;
+        case 2: batch_reduce_rows<2>(d, item.K, red); break;
+        case 3: batch_reduce_rows<3>(d, item.K, red); break;
+        case 4: batch_reduce_rows<4>(d, item.K, red); break;
+        case 5: batch_reduce_rows<5>(d, item.K, red); break;
+        case 6: batch_reduce_rows<6>(d, item.K, red); break;
+        case 8: batch_reduce_rows<8>(d, item.K, red); break;
+        case 10: batch_reduce_rows<10>(d, item.K, red); break;
+        default: break;  // the host only admits kernel dimensions <= 10 here
     }
 }
 

@@ -244,6 +244,7 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
     };
 
     load_group(bufA, bufAT, kFirst);
+#pragma nounroll
     for (int k0 = kFirst; k0 <= kLast; k0 += 2 * G) {
         load_group(bufB, bufBT, k0 + G);
         __builtin_amdgcn_sched_barrier(0);
