@@ -536,6 +536,10 @@ def main():
                     help="only run a plain device copy of MIB MiB (read + write), to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a "
+              "design, not a measured path", file=sys.stderr)
+        sys.exit(2)
     if args.calibrate_copy:
         _calibrate_copy(args.calibrate_copy)
         return
@@ -557,10 +561,6 @@ def main():
     if args.workload == "beast372":
         _bench_beast()
         return
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a "
-              "design, not a measured path", file=sys.stderr)
-        sys.exit(2)
 
     n, p, tau, delta, x = _workload()
     r64 = _run_gpu(n, p, tau, delta, x, False, args.steps, args.warmup, args.settle_ms)
