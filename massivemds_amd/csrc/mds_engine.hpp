@@ -111,8 +111,7 @@ struct Engine {
     int pendingRow = -1;
     double baseSum = 0.0;
     int rowUpdatesInARow = 0;
-    double* dRowPartial = nullptr;
-    double* hRowPartial = nullptr;  // pinned
+    double* hRowPartial = nullptr;  // pinned, coherent: the row kernel's per-workgroup partials
     unsigned int* dRowDone = nullptr;  // the row kernel's completion counter
     // A single-row updateLocations does not touch the device: the location
     // array keeps the row's previous value, which the row kernel reads
@@ -174,8 +173,8 @@ struct Engine {
             check(hipHostMalloc((void**)&hSum, sizeof(double), hipHostMallocDefault), "hipHostMalloc(sum)");
             check(hipHostMalloc(&hX, (size_t)npad * kdim * realSize, hipHostMallocDefault),
                   "hipHostMalloc(locations)");
-            check(hipMalloc((void**)&dRowPartial, row_delta_blocks(N) * sizeof(double)), "hipMalloc(row sums)");
-            check(hipHostMalloc((void**)&hRowPartial, row_delta_blocks(N) * sizeof(double), hipHostMallocDefault),
+            // written by the row kernel itself (coherent pinned memory, no copy)
+            check(hipHostMalloc((void**)&hRowPartial, row_delta_blocks(N) * sizeof(double), hipHostMallocCoherent),
                   "hipHostMalloc(row sums)");
             check(hipMalloc((void**)&dRowDone, sizeof(unsigned int)), "hipMalloc(row counter)");
             check(hipMemsetAsync(dRowDone, 0, sizeof(unsigned int), stream), "hipMemset(row counter)");
@@ -208,12 +207,11 @@ struct Engine {
         (void)hipHostFree(hGradStored);
         (void)hipHostFree(hSum);
         (void)hipHostFree(hX);
-        (void)hipFree(dRowPartial);
         (void)hipHostFree(hRowPartial);
         (void)hipFree(dRowDone);
         dRowDone = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
-        dRowPartial = hRowPartial = nullptr;
+        hRowPartial = nullptr;
         dDelta = dX[0] = dX[1] = nullptr;
         dTiles = nullptr;
         dTileSums = dPart = dGrad = dSum = hGrad = hGradStored = hSum = nullptr;
@@ -316,12 +314,10 @@ struct Engine {
         a.truncated = truncated;
         a.symmetric = symmetric;
         a.fp32 = fp32;
-        a.partial = dRowPartial;
+        a.partial = hRowPartial;
         check(launch_row_delta(a, stream), "row update kernel");
         rowWritePending = false;  // the kernel's last workgroup writes the row
         const int blocks = row_delta_blocks(N);
-        check(hipMemcpyAsync(hRowPartial, dRowPartial, blocks * sizeof(double), hipMemcpyDeviceToHost, stream),
-              "hipMemcpy(row sums)");
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         double change = 0.0;
         for (int b = 0; b < blocks; ++b) change += hRowPartial[b];

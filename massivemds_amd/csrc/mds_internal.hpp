@@ -58,8 +58,9 @@ struct ReduceLaunch {
 // Change of the raw sum of increments when one location row moves (O(N)):
 // per block of kRowThreads locations j, the partial
 // sum_j [term(delta_ij, d_new) - term(delta_ij, d_old) + the same for delta_ji]
-// over observed entries, written to partial[block] (the caller adds the
-// partials in order and halves).
+// over observed entries, written to partial[block] (device-accessible; the
+// engine passes coherent pinned host memory) -- the caller adds the
+// partials in order and halves.
 constexpr int kRowThreads = 256;
 
 struct RowDeltaLaunch {

@@ -49,8 +49,9 @@ __device__ __forceinline__ double pair_term(double obs, double d, double halfTau
 // before it signals completion); the last workgroup to finish -- after all
 // the others' reads -- writes the new location into x and resets the
 // counter for the next launch.  The counter orders the write only; the
-// change itself is summed per workgroup in a fixed order and the partials
-// are added on the host in order.
+// change itself is summed per workgroup in a fixed order, each workgroup
+// stores its partial straight into coherent pinned host memory, and the
+// host adds them in order after the stream synchronises.
 template <typename Real, int P>
 __global__ __launch_bounds__(kRowThreads) void row_delta_kernel(RowDeltaLaunch a) {
     const Real* delta = (const Real*)a.delta;
@@ -84,8 +85,8 @@ __global__ __launch_bounds__(kRowThreads) void row_delta_kernel(RowDeltaLaunch a
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int k = 0; k < kRowThreads / kWave; ++k) s += waveSums[k];
-        a.partial[blockIdx.x] = s;
-        __threadfence();
+        a.partial[blockIdx.x] = s;  // pinned host memory
+        __threadfence_system();
         const unsigned int prev = atomicAdd(a.done, 1u);
         if (prev == gridDim.x - 1) {
             __threadfence();
