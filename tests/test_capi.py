@@ -2,7 +2,10 @@
 function include/mds_hip.h declares, and fails cleanly (status + message, no
 crash, no CPU fallback) where no GPU is present."""
 import ctypes
+import os
 import re
+import subprocess
+import sys
 from pathlib import Path
 
 import pytest
@@ -107,3 +110,21 @@ def test_singleton_harness_builds(tmp_path):
 
     exe = harness(tmp_path)
     assert exe.exists()
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the built HIP library the engine raises at construction: there
+    is no CPU fallback to run instead."""
+    code = ("import numpy as np\n"
+            "from massivemds_amd.engine import Flags, MultiDimensionalScaling\n"
+            "try:\n"
+            "    MultiDimensionalScaling(2, 10, Flags.OPENCL, 0)\n"
+            "except RuntimeError as e:\n"
+            "    print('raised:', e)\n"
+            "else:\n"
+            "    print('no error')\n")
+    env = dict(os.environ, MDS_HIP_LIBRARY=str(tmp_path / "absent" / "libmds_hip.so"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120, env=env,
+                       cwd=Path(__file__).resolve().parents[1])
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("raised:") and "is missing" in r.stdout
