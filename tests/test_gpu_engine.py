@@ -77,7 +77,8 @@ def test_fp64_parity(oracle, n, p, pattern, truncated):
     assert sym == (pattern != "asym")
 
 
-@pytest.mark.parametrize("n,p,pattern", [(257, 2, "sym5"), (1000, 2, "sym5"), (130, 5, "asym"), (200, 10, "none")])
+@pytest.mark.parametrize("n,p,pattern", [(257, 2, "sym5"), (1000, 2, "sym5"), (130, 5, "asym"), (200, 10, "none"),
+                                         (70, 12, "row"), (129, 16, "asym")])
 @pytest.mark.parametrize("truncated", [True, False])
 def test_fp32_error_against_fp64_oracle(oracle, n, p, pattern, truncated):
     tau = 2.0

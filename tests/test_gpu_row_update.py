@@ -32,7 +32,8 @@ def _fresh_sum(n, p, truncated, fp32, delta, x, tau=1.7):
 
 @pytest.mark.parametrize("n,p,truncated,asym,fp32", [
     (300, 2, True, False, False), (300, 2, False, False, False), (257, 3, True, True, False),
-    (130, 5, True, False, False), (1000, 2, True, True, False), (300, 2, True, False, True)])
+    (130, 5, True, False, False), (1000, 2, True, True, False), (300, 2, True, False, True),
+    (129, 16, True, True, False), (90, 12, False, False, True)])
 def test_single_row_updates_match_recomputation(n, p, truncated, asym, fp32):
     print(f"row updates n={n} p={p} T={truncated} asym={asym} fp32={fp32}", flush=True)
     delta = datagen.euclid10(n, 21, 0.1, 0.05)
