@@ -296,6 +296,24 @@ def _bench_hmc(iterations=2000, burn_in=500, n=5000, p=2):
     print(json.dumps(out))
 
 
+def _bench_leapfrog(steps, warmup, settle_ms, n=5000, p=2, tau=1.0):
+    """The gradient pass of one leapfrog step of the HMC workload (N = 5000,
+    P = 2, FP64, euclid10 with 2% missing, truncation, the sampler's starting
+    precision), launched directly instead of from the sampler's graph, so
+    that its kernels can be traced: one fused likelihood + gradient
+    evaluation per step, inputs resident."""
+    from massivemds_amd import datagen
+
+    delta = datagen.euclid10(n, missing=0.02)
+    x = datagen.locations(n, p)
+    r = _run_gpu(n, p, tau, delta, x, False, steps, warmup, settle_ms)
+    print(json.dumps({
+        "workload": f"leapfrog_gradient_n{n}_p{p}_fp64_trunc_missing2", "steps": steps, "warmup": warmup,
+        "settle": {"ms": r["settle_ms"], "evaluations": r["settle_evaluations"]},
+        "evals_per_s": 1e3 / r["wall_ms_per_step"], "us_per_eval": r["wall_ms_per_step"] * 1e3,
+        "tile_kernel_us": r["tile_ms"] * 1e3, "eval_us_events": r["eval_ms"] * 1e3, "tiles": r["tiles"]}))
+
+
 def _bench_surrogate(iterations=500, n=20000, p=5):
     """Surrogate-trajectory HMC against plain HMC: N = 20000, P = 5, FP64,
     euclid10 with 2% missing, truncation, precision fixed at its plug-in value
@@ -521,13 +539,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--settle-ms", type=float, default=100.0,
                     help="untimed evaluations after the warm-up for this many ms of wall time (the GPU clock "
-                         "settles under sustained load), before each timed pass of the default workload; 0: none")
+                         "settles under sustained load), before each timed pass of the default and leapfrog5000 "
+                         "workloads; 0: none")
     ap.add_argument("--workload", default="n10000",
-                    choices=["n10000", "n1000", "batch64", "hmc5000", "surrogate20000", "rowupdate10000",
-                             "beast372"],
+                    choices=["n10000", "n1000", "batch64", "hmc5000", "leapfrog5000", "surrogate20000",
+                             "rowupdate10000", "beast372"],
                     help="n10000: the metric's workload (default); n1000: the reference benchmark's loop "
                          "at N = 1000 (fresh locations per iteration, through the API); batch64: the uneven batch; "
-                         "hmc5000: plain HMC; surrogate20000: surrogate against plain HMC; "
+                         "hmc5000: plain HMC; leapfrog5000: the HMC workload's per-step gradient pass, launched "
+                         "directly (for kernel traces); surrogate20000: surrogate against plain HMC; "
                          "rowupdate10000: single-location moves through the O(N) update; "
                          "beast372: real BEAST data (N = 372) against the logged likelihood")
     ap.add_argument("--iterations", type=int, default=500, help="sampling iterations (surrogate20000)")
@@ -551,6 +571,9 @@ def main():
         return
     if args.workload == "hmc5000":
         _bench_hmc()
+        return
+    if args.workload == "leapfrog5000":
+        _bench_leapfrog(args.steps, args.warmup, args.settle_ms)
         return
     if args.workload == "surrogate20000":
         _bench_surrogate(args.iterations)
