@@ -38,21 +38,43 @@ HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
 
 
-def _measured_traffic():
-    """HBM bytes per FP64 tile-kernel launch at this workload, from the
-    rocprofv3 FETCH_SIZE + WRITE_SIZE runs summarised in
-    profiles/tile_kernel_n10000_p2.md (counters scaled by a plain copy's
-    reading); None when the summary is absent."""
+def _profile_figure(key):
+    """A per-launch figure of the FP64 tile kernel at this workload from the
+    rocprofv3 runs summarised in profiles/tile_kernel_n10000_p2.md; None when
+    the summary is absent."""
     from pathlib import Path
 
     path = Path(__file__).resolve().parent / "profiles" / "tile_kernel_n10000_p2.md"
     try:
         for line in path.read_text().splitlines():
-            if line.startswith("traffic_bytes_fp64_tile:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def _measured_traffic():
+    """HBM bytes per FP64 tile-kernel launch (FETCH_SIZE + WRITE_SIZE, scaled
+    by a plain copy's reading)."""
+    return _profile_figure("traffic_bytes_fp64_tile")
+
+
+# VALU issue ceiling: 256 CUs x 4 SIMDs, one wave64 instruction per 4 cycles
+# per SIMD, at the 2.4 GHz peak engine clock.
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def _valu_roofline(tile_s):
+    """The tile kernel against its VALU issue ceiling: wave instructions per
+    launch (SQ_INSTS_VALU from the profile) over the measured duration."""
+    instr = _profile_figure("valu_wave_instructions_fp64_tile")
+    if instr is None:
+        return None
+    achieved = instr / tile_s
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_WAVE_INSTR_PER_S,
+            "unit": "wave instructions/s", "frac": achieved / VALU_PEAK_WAVE_INSTR_PER_S,
+            "instructions": instr}
 
 
 def _workload():
@@ -623,6 +645,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": _measured_traffic(),
         },
+        "roofline_valu": _valu_roofline(r64["tile_ms"] * 1e-3),
         "fp32": {
             "value": 1e3 / r32["wall_ms_per_step"],
             "ms_per_step": r32["wall_ms_per_step"],
