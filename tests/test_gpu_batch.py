@@ -111,3 +111,26 @@ def test_batch_errors():
     with pytest.raises(MdsError):
         b.updateLocations(0, np.zeros(10))
     b.close()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_tiny_and_ragged_datasets(fp32):
+    """Datasets of 1, 2, 63, 64 and 65 locations (one row block, one
+    location, a block boundary on either side) and P from 1 to 10 in one
+    batch, truncated and not: each equals the single-instance engine bit for
+    bit (N = 1 has no pair: sum 0, gradient 0)."""
+    specs = [(1, 2, 1.3, 31), (2, 1, 0.7, 32), (63, 3, 2.0, 33), (64, 10, 1.1, 34), (65, 5, 3.0, 35),
+             (2, 10, 1.9, 36)]
+    for truncated in (True, False):
+        flags = Flags.OPENCL | (Flags.LEFT_TRUNCATION if truncated else 0) | (Flags.FLOAT if fp32 else 0)
+        data = [(*datagen.batch_dataset(n, p, seed), tau) for (n, p, tau, seed) in specs]
+        print(f"tiny batch fp32={fp32} truncated={truncated}", flush=True)
+        b = _batch(specs, data, flags)
+        b.evaluate()
+        sums = b.sums()
+        for k, (delta, x, tau) in enumerate(data):
+            s, g = _single(delta, x, tau, flags)
+            assert sums[k] == s, (k, sums[k], s)
+            np.testing.assert_array_equal(b.gradient(k), g)
+        assert sums[0] == 0.0 and np.all(b.gradient(0) == 0.0)
+        b.close()
