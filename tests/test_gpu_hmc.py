@@ -104,3 +104,37 @@ def test_learning_smoke(truncation):
     if not truncation:
         assert res["precision_acceptance"] == 1.0  # Gibbs
     assert res["trace"][:, 2].std() > 0  # trait precision moves
+
+
+def test_argument_and_state_errors():
+    """The sampler refuses to start without data or a precision, and rejects
+    bad iteration counts, trajectories, trait-update weights, initial
+    locations and a surrogate mode without a surrogate, each with a status
+    and a message; the engine still samples afterwards."""
+    from massivemds_amd.engine import MdsError
+
+    n, p = 40, 2
+    delta = datagen.euclid10(n, seed=5)
+    x0 = datagen.locations(n, p, seed=6)
+    e = MultiDimensionalScaling(p, n, Flags.OPENCL | Flags.LEFT_TRUNCATION, 0)
+    with pytest.raises(MdsError) as info:
+        hmc.run_hmc(e, x0, _options(iterations=4, burnIn=1))
+    assert info.value.status == 3 and "data" in str(info.value)
+    e.setPairwiseData(delta)
+    with pytest.raises(MdsError) as info:
+        hmc.run_hmc(e, x0, _options(iterations=4, burnIn=1))
+    assert info.value.status == 3 and "precision" in str(info.value)
+    e.setParameters([1.5])
+    bad = [dict(iterations=0, burnIn=0), dict(iterations=4, burnIn=4), dict(iterations=4, burnIn=1, leapfrogSteps=0),
+           dict(iterations=4, burnIn=1, trajectory=0.0), dict(iterations=4, burnIn=1, traitInvWeight=0),
+           dict(iterations=4, burnIn=1, useSurrogate=1)]
+    for kw in bad:
+        with pytest.raises(MdsError) as info:
+            hmc.run_hmc(e, x0, _options(**kw))
+        assert info.value.status == 1, kw
+    with pytest.raises(MdsError) as info:
+        hmc.run_hmc(e, x0[:-1], _options(iterations=4, burnIn=1))
+    assert info.value.status == 1
+    trace, final, stats = hmc.run_hmc(e, x0, _options(iterations=4, burnIn=1, leapfrogSteps=5, trajectory=0.05))
+    assert trace.shape == (3, len(hmc.TRACE_COLUMNS)) and np.all(np.isfinite(trace)) and np.all(np.isfinite(final))
+    e.close()
