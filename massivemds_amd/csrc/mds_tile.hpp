@@ -194,6 +194,8 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
     constexpr int kFirst = DIAG ? 1 : 0;                 // first step
     constexpr int kSteps = DIAG ? kTile / 2 : kTile;     // steps walked
     constexpr int kLast = kFirst + kSteps - 1;
+    constexpr int kFlushSteps = 4 * G;                   // steps per compensated update (FP32)
+    static_assert(kSteps % kFlushSteps == 0, "the walk must end on a flush");
     const int lane = threadIdx.x % kWave;
     const Real* tileStep = tileBase + lane;
     // Two register buffers of G steps each (ping-pong): while one group is
@@ -252,12 +254,16 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
         load_group(bufA, bufAT, k0 + 2 * G);
         __builtin_amdgcn_sched_barrier(0);
         run_group(bufB, bufBT, k0 + G);
-        // close the group of 2 G steps (FP32: one compensated update per sum)
-        acc.s.flush();
+        // every second pass (4 G steps) the FP32 group sums enter their
+        // compensated sums (FP64: nothing to do); the walk's step count is a
+        // multiple of 4 G, so its last pass flushes
+        if (((k0 - kFirst) / (2 * G)) % 2 == 1) {
+            acc.s.flush();
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            acc.row[p].flush();
-            acc.col[p].template flush<2 * G>();
+            for (int p = 0; p < P; ++p) {
+                acc.row[p].flush();
+                acc.col[p].template flush<kFlushSteps>();
+            }
         }
     }
 
