@@ -77,10 +77,10 @@ def _valu_roofline(tile_s):
             "instructions": instr}
 
 
-def _workload():
+def _workload(n=10000):
     from massivemds_amd import datagen
 
-    n, p, tau = 10000, 2, 2.0
+    p, tau = 2, 2.0
     delta = datagen.euclid10(n, noise_sd=0.1, missing=0.05)
     x = datagen.locations(n, p)
     return n, p, tau, delta, x
@@ -563,6 +563,9 @@ def main():
                     help="untimed evaluations after the warm-up for this many ms of wall time (the GPU clock "
                          "settles under sustained load), before each timed pass of the default and leapfrog5000 "
                          "workloads; 0: none")
+    ap.add_argument("--n", type=int, default=10000,
+                    help="locations for the default workload (the metric is quoted at 10000; other sizes "
+                         "for scaling studies)")
     ap.add_argument("--workload", default="n10000",
                     choices=["n10000", "n1000", "batch64", "hmc5000", "leapfrog5000", "surrogate20000",
                              "rowupdate10000", "beast372"],
@@ -607,7 +610,7 @@ def main():
         _bench_beast()
         return
 
-    n, p, tau, delta, x = _workload()
+    n, p, tau, delta, x = _workload(args.n)
     r64 = _run_gpu(n, p, tau, delta, x, False, args.steps, args.warmup, args.settle_ms)
     r32 = _run_gpu(n, p, tau, delta, x, True, args.steps, args.warmup, args.settle_ms)
 
@@ -635,7 +638,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (euclid10 distances + N(0,0.1^2) noise, 5% symmetric NaN; portable PRNG seeds 666/667)",
-        "config": {"workload": "mds_loglik_grad_n10000_p2_trunc_tau2_missing5", "N": n, "P": p,
+        "config": {"workload": f"mds_loglik_grad_n{n}_p2_trunc_tau2_missing5", "N": n, "P": p,
                    "precision": tau, "truncation": True, "missing": 0.05},
         "roofline": {
             "bound": "hbm",
@@ -643,9 +646,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": _measured_traffic(),
+            "traffic": _measured_traffic() if n == 10000 else None,  # profiled at N = 10000
         },
-        "roofline_valu": _valu_roofline(r64["tile_ms"] * 1e-3),
+        "roofline_valu": _valu_roofline(r64["tile_ms"] * 1e-3) if n == 10000 else None,
         "fp32": {
             "value": 1e3 / r32["wall_ms_per_step"],
             "ms_per_step": r32["wall_ms_per_step"],
