@@ -9,8 +9,11 @@
  * Every call returns MDS_HIP_OK or an MDS_HIP_ERR_* status (mds_hip.h);
  * mds_hip_singleton_last_error(instance) gives the message (instance -1: the
  * registry's own errors, such as a bad instance number).  The registry is safe
- * to use from several threads; one instance is used by one thread at a time,
- * as in the reference.
+ * to use from several threads: calls on one instance are serialised (each
+ * holds the instance's lock), mds_hip_singleton_release_all waits for calls
+ * in flight, and a call on a released instance returns MDS_HIP_ERR_ARGUMENT.
+ * mds_hip_singleton_last_error returns the calling thread's copy of the
+ * message, valid until that thread calls it again.
  */
 #ifndef MDS_HIP_SINGLETON_H
 #define MDS_HIP_SINGLETON_H

@@ -10,10 +10,20 @@
 #ifndef MDS_INTERFACE_HPP
 #define MDS_INTERFACE_HPP
 
+#include <climits>
 #include <cstddef>
 #include <memory>
 
 namespace mds {
+
+// N (N - 1) / 2 in 64-bit arithmetic, saturated to INT_MAX: the member
+// stays an int as in the reference, whose int product overflows for
+// N > 46,341 (src/AbstractMultiDimensionalScaling.hpp:42).
+inline int observationCountOf(int locationCount) {
+    const long long n = locationCount;
+    const long long pairs = n * (n - 1) / 2;
+    return pairs > INT_MAX ? INT_MAX : (int)pairs;
+}
 
 enum Flags {
     FLOAT = 1 << 2,
@@ -31,7 +41,7 @@ public:
     AbstractMultiDimensionalScaling(int embeddingDimension, int locationCount, long flags)
         : embeddingDimension(embeddingDimension),
           locationCount(locationCount),
-          observationCount(locationCount * (locationCount - 1) / 2),
+          observationCount(observationCountOf(locationCount)),
           flags(flags) {}
 
     virtual ~AbstractMultiDimensionalScaling() = default;

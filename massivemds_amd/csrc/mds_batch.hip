@@ -33,6 +33,7 @@
 #include "mds_host.hpp"
 #include "mds_internal.hpp"
 #include "mds_tile.hpp"
+#include "mds_tune_cache.hpp"
 
 namespace mdship {
 
@@ -615,18 +616,11 @@ struct Batch {
     double autotune(const char* cachePath, int force) {
         const std::string sig = signature();
         if (cachePath && !force) {
-            std::ifstream in(cachePath);
-            std::string line;
-            while (std::getline(in, line)) {
-                std::istringstream ls(line);
-                std::string s;
-                BatchConfig c;
-                double ms = 0;
-                if (ls >> s >> c.wavesPerBlock >> c.blocksPerCU >> c.tilesPerItem >> c.strategy >> ms && s == sig) {
-                    cfg = c;
-                    planDirty = true;
-                    return ms;
-                }
+            TuneRecord r;
+            if (tune_cache_find(cachePath, sig, 4, r)) {
+                cfg = {(int)r.fields[0], (int)r.fields[1], (int)r.fields[2], (int)r.fields[3]};
+                planDirty = true;
+                return r.ms;
             }
         }
         BatchConfig best = cfg;
@@ -647,10 +641,10 @@ struct Batch {
         cfg = best;
         planDirty = true;
         if (cachePath) {
-            std::ofstream out(cachePath, std::ios_base::app);
-            out.precision(17);  // the time reads back exactly
-            out << sig << " " << cfg.wavesPerBlock << " " << cfg.blocksPerCU << " " << cfg.tilesPerItem << " "
-                << cfg.strategy << " " << bestMs << "\n";
+            TuneRecord r;
+            r.fields = {cfg.wavesPerBlock, cfg.blocksPerCU, cfg.tilesPerItem, cfg.strategy};
+            r.ms = bestMs;
+            (void)tune_cache_store(cachePath, sig, r);
         }
         return bestMs;
     }

@@ -108,6 +108,12 @@ struct Engine {
     // moved is baseSum
     static constexpr int kMaxRowUpdates = 1024;
     bool sumFromCurrentData = false;  // the known sum was computed on the current observations
+    // storeState saved a sum that was known and computed on the current
+    // observations; new observations, makeDirty and a truncated precision
+    // change clear it.  restoreState always restores the stored sum as the
+    // known sum (src/NewMultiDimensionalScaling.hpp:234-236), but only a
+    // valid one may serve as the base of a later O(N) single-row update.
+    bool storedSumValid = false;
     int pendingRow = -1;
     double baseSum = 0.0;
     int rowUpdatesInARow = 0;
@@ -360,6 +366,7 @@ struct Engine {
         symmetric = sym;
         dataSet = true;
         sumFromCurrentData = false;  // the cached sum stays (:257-260) but is no base for row updates
+        storedSumValid = false;
         pendingRow = -1;
         storedGradFresh = false;
         storedSharesCurrent = false;
@@ -377,6 +384,7 @@ struct Engine {
         if (truncated) {
             sumKnown = false;
             pendingRow = -1;  // the base sum was for another precision
+            storedSumValid = false;
         }
         storedGradFresh = false;
         storedSharesCurrent = false;
@@ -386,6 +394,7 @@ struct Engine {
     void makeDirty() {  // :276-279
         sumKnown = false;
         pendingRow = -1;
+        storedSumValid = false;
         storedGradFresh = false;
         storedSharesCurrent = false;
         invalidate_device_results();
@@ -394,6 +403,7 @@ struct Engine {
     void storeState() {  // :205-217
         flush_row_write();
         storedSumOfIncrements = sumOfIncrements;
+        storedSumValid = sumKnown && sumFromCurrentData;
         storedGradFresh = hostGradFresh;
         storedSharesCurrent = hostGradFresh;
         check(hipMemcpyAsync(dX[1 - cur], dX[cur], (size_t)npad * kdim * realSize,
@@ -406,6 +416,7 @@ struct Engine {
         flush_row_write();
         sumOfIncrements = storedSumOfIncrements;
         sumKnown = true;
+        sumFromCurrentData = storedSumValid;
         pendingRow = -1;
         precision = storedPrecision;
         cur = 1 - cur;
@@ -429,20 +440,12 @@ struct Engine {
         return e;
     }
 
-    // devTau: when non-null, the kernels read the precision from this device
-    // address instead of the engine's value (for captured graphs whose
-    // precision changes between replays).
-    void evaluate(bool gradient, const double* devTau = nullptr) {
-        if (!dataSet) throw Error(MDS_HIP_ERR_STATE, "pairwise data not set");
-        if (!precisionSet) throw Error(MDS_HIP_ERR_STATE, "precision not set");
-        flush_row_write();
-        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
-        if (timing) {
-            e0 = take_event();
-            e1 = take_event();
-            e2 = take_event();
-            check(hipEventRecord(e0, stream), "hipEventRecord");
-        }
+    // The tile kernel of one evaluation at the current device locations into
+    // the given tile-sum and partial buffers.  devTau: when non-null, the
+    // kernels read the precision from this device address instead of the
+    // engine's value (for captured graphs whose precision changes between
+    // replays).
+    void launch_tile_pass(bool gradient, const double* devTau, double* tileSums, double* part) {
         TileLaunch a;
         a.delta = dDelta;
         a.x = dX[cur];
@@ -455,23 +458,45 @@ struct Engine {
         a.truncated = truncated;
         a.symmetric = symmetric;
         a.gradient = gradient;
-        a.tileSums = dTileSums;
-        a.part = dPart;
+        a.tileSums = tileSums;
+        a.part = part;
         a.devTau = devTau;
         check(fp32 ? launch_tiles_f32(a, stream) : launch_tiles_f64(a, stream), "tile kernel");
-        if (timing) check(hipEventRecord(e1, stream), "hipEventRecord");
+    }
+
+    // The reduce kernel: the gradient rows into grad (when gradient) and
+    // (0.5 * the sum of the tile sums) * sumScale into *sumOut.
+    void launch_reduce_pass(bool gradient, const double* tileSums, const double* part, double* grad, double* sumOut,
+                            double sumScale) {
         ReduceLaunch r;
-        r.part = dPart;
-        r.tileSums = dTileSums;
+        r.part = part;
+        r.tileSums = tileSums;
         r.numTiles = numTiles;
         r.nb = nb;
         r.kdim = kdim;
         r.n = N;
         r.p = P;
         r.gradient = gradient;
-        r.grad = dGrad;
-        r.sumOut = dSum;
+        r.grad = grad;
+        r.sumOut = sumOut;
+        r.sumScale = sumScale;
         check(launch_reduce(r, stream), "reduce kernel");
+    }
+
+    void evaluate(bool gradient, const double* devTau = nullptr) {
+        if (!dataSet) throw Error(MDS_HIP_ERR_STATE, "pairwise data not set");
+        if (!precisionSet) throw Error(MDS_HIP_ERR_STATE, "precision not set");
+        flush_row_write();
+        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+        if (timing) {
+            e0 = take_event();
+            e1 = take_event();
+            e2 = take_event();
+            check(hipEventRecord(e0, stream), "hipEventRecord");
+        }
+        launch_tile_pass(gradient, devTau, dTileSums, dPart);
+        if (timing) check(hipEventRecord(e1, stream), "hipEventRecord");
+        launch_reduce_pass(gradient, dTileSums, dPart, dGrad, dSum, 1.0);
         if (timing) {
             check(hipEventRecord(e2, stream), "hipEventRecord");
             pending.push_back(e0);

@@ -292,6 +292,7 @@ Hmc::Hmc(Engine& engine, const mds_hip_hmc_options& options) : e(engine), opt(op
     check(hipMalloc((void**)&dGram, kMaxDim * kMaxDim * sizeof(double)), "hipMalloc(gram)");
     check(hipHostMalloc((void**)&hRows, kTraceRows * kMaxDim * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
     check(hipHostMalloc((void**)&hGram, kMaxDim * kMaxDim * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+    check(hipEventCreateWithFlags(&uploadDone, hipEventDisableTiming), "hipEventCreate");
     {
         const int r[kTraceRows] = {0, 1 % n, n / 4, n / 3, n / 2, 2 * n / 3, 3 * n / 4, n - 1};
         for (int k = 0; k < kTraceRows; ++k) traceRows.r[k] = r[k];
@@ -329,14 +330,20 @@ Hmc::~Hmc() {
     (void)hipFree(dGram);
     (void)hipHostFree(hRows);
     (void)hipHostFree(hGram);
+    if (uploadDone) (void)hipEventDestroy(uploadDone);
 }
 
+// The pinned block is rewritten only after the previous upload from it has
+// run (the copy reads host memory when it executes, not when it is queued).
 void Hmc::uploadParams() {
+    if (uploadPending) check(hipEventSynchronize(uploadDone), "hipEventSynchronize(parameters)");
     hParams[kHmcTau] = tau;
     hParams[kHmcEps] = eps;
     for (int k = 0; k < p * p; ++k) hParams[kHmcTrait + k] = trait[k];
     check(hipMemcpyAsync(dParams, hParams, kHmcParamCount * sizeof(double), hipMemcpyHostToDevice, e.stream),
           "hipMemcpy(parameters)");
+    check(hipEventRecord(uploadDone, e.stream), "hipEventRecord(parameters)");
+    uploadPending = true;
 }
 
 template <int FORM>
@@ -382,16 +389,32 @@ void Hmc::gradient(int mode) {
         launch_snapshot(e, e.dGrad, sur->store.sx, sur->store.sg, sur->store.dSlot, sur->store.capacity, e.stream);
 }
 
-// The whole trajectory as one graph: one launch per iteration.
+// The whole trajectory as one graph: one launch per iteration.  Every exit
+// ends the capture and restores the engine's timing switch, so a failure
+// while capturing leaves the stream usable.
 void Hmc::buildGraph(int mode) {
     if (graphExec[mode]) return;
     const bool timing = e.timing;
     e.timing = false;
-    check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    trajectory(mode);
-    check(hipStreamEndCapture(e.stream, &graph[mode]), "hipStreamEndCapture");
-    check(hipGraphInstantiate(&graphExec[mode], graph[mode], nullptr, nullptr, 0), "hipGraphInstantiate");
+    const hipError_t begun = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal);
+    if (begun != hipSuccess) {
+        e.timing = timing;
+        check(begun, "hipStreamBeginCapture");
+    }
+    hipGraph_t g = nullptr;
+    try {
+        trajectory(mode);
+    } catch (...) {
+        if (hipStreamEndCapture(e.stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        e.timing = timing;
+        throw;
+    }
+    const hipError_t ended = hipStreamEndCapture(e.stream, &g);
     e.timing = timing;
+    check(ended, "hipStreamEndCapture");
+    graph[mode] = g;
+    check(hipGraphInstantiate(&graphExec[mode], graph[mode], nullptr, nullptr, 0), "hipGraphInstantiate");
 }
 
 // energies at the current locations: hScalars[0] = K, [1] = x^T T x sum,

@@ -35,6 +35,8 @@ struct Hmc {
     double* dParams = nullptr;
     double* dEnergy = nullptr;
     double* hParams = nullptr;
+    hipEvent_t uploadDone = nullptr;  // after the last copy from hParams
+    bool uploadPending = false;
     double* hScalars = nullptr;
     void* hX = nullptr;
     double* dRows = nullptr;  // trace rows [kTraceRows][p]

@@ -52,7 +52,8 @@ struct ReduceLaunch {
     int p;                   // output dimension (<= kdim)
     bool gradient;
     double* grad;            // [n][p]
-    double* sumOut;          // 1 double: 0.5 * sum of tile sums
+    double* sumOut;          // 1 double: (0.5 * sum of tile sums) * sumScale
+    double sumScale = 1.0;
 };
 
 // Change of the raw sum of increments when one location row moves (O(N)):

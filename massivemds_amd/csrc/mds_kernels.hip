@@ -53,11 +53,12 @@ constexpr int kReduceThreads = kReduceWaves * kWave;
 // Block b < nb: gradient rows of row block b, 8 wavefronts splitting the
 // column blocks J (J = w mod 8), combined through LDS in fixed order.
 // Last block: the likelihood, a fixed-order FP64 sum of the tile partials,
-// halved as in src/NewMultiDimensionalScaling.hpp:556.
+// halved as in src/NewMultiDimensionalScaling.hpp:556, then scaled by
+// sumScale (1, or 0.5 tau for the untruncated value of :198-202).
 template <int P>
 __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
     const double* __restrict__ part, const double* __restrict__ tileSums, int numTiles,
-    int nb, int n, int pOut, double* __restrict__ grad, double* __restrict__ sumOut) {
+    int nb, int n, int pOut, double* __restrict__ grad, double* __restrict__ sumOut, double sumScale) {
     __shared__ double red[kReduceThreads * (P > 1 ? P : 1)];
     const int tid = threadIdx.x;
     // Loads are issued kAhead iterations at a time and added in the same
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
             if (tid < s) red[tid] += red[tid + s];
             __syncthreads();
         }
-        if (tid == 0) *sumOut = 0.5 * red[0];
+        if (tid == 0) *sumOut = (0.5 * red[0]) * sumScale;  // x * 1.0 == x: the plain sum keeps its bits
         return;
     }
     const int K = blockIdx.x;
@@ -169,7 +170,7 @@ template <int P>
 static hipError_t reduce_p(const ReduceLaunch& a, hipStream_t s) {
     const dim3 grid(a.gradient ? a.nb + 1 : 1);
     hipLaunchKernelGGL((reduce_kernel<P>), grid, dim3(kReduceThreads), 0, s, a.part, a.tileSums,
-                       a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut);
+                       a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut, a.sumScale);
     return hipGetLastError();
 }
 

@@ -13,33 +13,53 @@
 
 namespace {
 
+// One instance.  Its mutex is held for the whole of every call on it and by
+// release, so a release on another thread waits for a call in flight and a
+// later call finds the handle gone; the error message is written and read
+// under it.  Slots are shared, so a caller that found one keeps it alive.
 struct Slot {
+    std::mutex m;
     mds_hip_handle h = nullptr;
     std::string error = "no error";
 };
 
-std::mutex g_mutex;
-std::vector<std::unique_ptr<Slot>> g_slots;
+std::mutex g_mutex;  // the slot list and the registry's own error
+std::vector<std::shared_ptr<Slot>> g_slots;
 std::string g_error = "no error";
 
-// The slot of an instance number, or null (with the registry's error set).
-Slot* slot_of(int instance) {
+void set_registry_error(const std::string& message) {
     std::lock_guard<std::mutex> lock(g_mutex);
-    if (instance < 0 || (size_t)instance >= g_slots.size() || !g_slots[instance] || !g_slots[instance]->h) {
+    g_error = message;
+}
+
+// The slot of an instance number, or null (with the registry's error set).
+std::shared_ptr<Slot> slot_of(int instance) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (instance < 0 || (size_t)instance >= g_slots.size() || !g_slots[instance]) {
         g_error = "no instance " + std::to_string(instance);
         return nullptr;
     }
-    return g_slots[instance].get();
+    return g_slots[instance];
 }
 
 template <typename F>
 int call(int instance, F&& f) {
-    Slot* s = slot_of(instance);
+    const std::shared_ptr<Slot> s = slot_of(instance);
     if (!s) return MDS_HIP_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lock(s->m);
+    if (!s->h) {
+        s->error = "instance " + std::to_string(instance) + " was released";
+        set_registry_error(s->error);
+        return MDS_HIP_ERR_ARGUMENT;
+    }
     const int st = f(s->h);
     if (st != MDS_HIP_OK) s->error = mds_hip_last_error(s->h);
     return st;
 }
+
+// The returned message is this thread's copy, valid until its next call of
+// mds_hip_singleton_last_error.
+thread_local std::string t_message;
 
 }  // namespace
 
@@ -65,7 +85,7 @@ int mds_hip_singleton_initialize(int embeddingDimension, int elementCount, long 
         g_error = mds_hip_last_error(nullptr);
         return st;
     }
-    auto s = std::make_unique<Slot>();
+    auto s = std::make_shared<Slot>();
     s->h = h;
     g_slots.push_back(std::move(s));
     *instance = (int)g_slots.size() - 1;
@@ -123,19 +143,35 @@ int mds_hip_singleton_instance_count(int* count) {
 }
 
 int mds_hip_singleton_release_all(void) {
-    std::lock_guard<std::mutex> lock(g_mutex);
-    for (auto& s : g_slots)
-        if (s && s->h) {
+    std::vector<std::shared_ptr<Slot>> slots;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        slots = g_slots;
+    }
+    for (auto& s : slots) {
+        if (!s) continue;
+        std::lock_guard<std::mutex> lock(s->m);  // waits for a call in flight on this instance
+        if (s->h) {
             mds_hip_destroy(s->h);
             s->h = nullptr;
         }
+    }
     return MDS_HIP_OK;
 }
 
 const char* mds_hip_singleton_last_error(int instance) {
-    std::lock_guard<std::mutex> lock(g_mutex);
-    if (instance < 0 || (size_t)instance >= g_slots.size() || !g_slots[instance]) return g_error.c_str();
-    return g_slots[instance]->error.c_str();
+    std::shared_ptr<Slot> s;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        if (instance < 0 || (size_t)instance >= g_slots.size() || !g_slots[instance]) {
+            t_message = g_error;
+            return t_message.c_str();
+        }
+        s = g_slots[instance];
+    }
+    std::lock_guard<std::mutex> lock(s->m);
+    t_message = s->error;
+    return t_message.c_str();
 }
 
 }  // extern "C"

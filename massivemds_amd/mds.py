@@ -35,16 +35,20 @@ class Engine:
 
 def createEngine(embeddingDimension: int, locationCount: int, truncation: bool, tbb: int = 0,
                  simd: int = 0, gpu: int = 1, single: bool = False) -> Engine:
-    """src/MassiveMDS.cpp:60-114.  ``gpu > 0`` sets the GPU flag with device
-    number ``gpu``; ``single`` selects the FP32 engine.  CPU engines (gpu = 0)
-    belong to the reference library and are not provided here."""
+    """src/MassiveMDS.cpp:60-114.  ``gpu > 0`` sets the GPU flag and selects
+    HIP device ``gpu - 1`` (counting GPUs from 1, as the reference's CLI does
+    with ``--gpu k``, src/benchmark.cpp:89-93; the R wrapper passes ``gpu``
+    itself, which on a one-GPU machine only reached device 0 through the
+    engine's out-of-range fallback); ``single`` selects the FP32 engine.  CPU
+    engines (gpu = 0) belong to the reference library and are not provided
+    here."""
     flags = 0
     if truncation:
         flags |= Flags.LEFT_TRUNCATION
     if gpu <= 0:
         raise ValueError("this library provides the GPU engine only: call createEngine(..., gpu=1)")
     flags |= Flags.OPENCL
-    device = int(gpu)
+    device = int(gpu) - 1
     if single:
         flags |= Flags.FLOAT
     eng = factory(embeddingDimension, locationCount, flags, device, 0)

@@ -1,10 +1,12 @@
 """GPU tests of the O(N) single-location update of the sum of increments
 (mds_row.hip, Engine::applyRowUpdate).  The reference disables this path
 (src/NewMultiDimensionalScaling.hpp:563-566) and recomputes; the engine must
-give the recomputed value (FP64: 1e-12 relative; FP32: its own full
-evaluation's accuracy) and keep the reference's state rules: a second row
-forces a full recomputation (:150-153), store/restore behave as for full
-updates (:205-255)."""
+give the recomputed value -- checked against the serial FP64 oracle at the
+moved locations after every move: FP64 within 1e-12 relative, FP32 within
+2e-6 -- and keep the reference's state rules: a second row forces a full
+recomputation (:150-153), store/restore behave as for full updates
+(:205-255), and a restored sum that was stale when stored is never the base
+of an O(N) update."""
 import numpy as np
 import pytest
 
@@ -23,18 +25,16 @@ def _engine(n, p, truncated, fp32, delta, x, tau=1.7):
     return e
 
 
-def _fresh_sum(n, p, truncated, fp32, delta, x, tau=1.7):
-    e = _engine(n, p, truncated, fp32, delta, x, tau)
-    s = e.getSumOfIncrements()
-    e.close()
-    return s
+def _oracle_sum(oracle, truncated, delta, x, tau=1.7):
+    """The serial FP64 oracle's getSumOfIncrements at locations x."""
+    return oracle.sum_of_increments(delta, x, tau, truncated)
 
 
 @pytest.mark.parametrize("n,p,truncated,asym,fp32", [
     (300, 2, True, False, False), (300, 2, False, False, False), (257, 3, True, True, False),
     (130, 5, True, False, False), (1000, 2, True, True, False), (300, 2, True, False, True),
     (129, 16, True, True, False), (90, 12, False, False, True)])
-def test_single_row_updates_match_recomputation(n, p, truncated, asym, fp32):
+def test_single_row_updates_match_recomputation(oracle, n, p, truncated, asym, fp32):
     print(f"row updates n={n} p={p} T={truncated} asym={asym} fp32={fp32}", flush=True)
     delta = datagen.euclid10(n, 21, 0.1, 0.05)
     if asym:
@@ -54,14 +54,14 @@ def test_single_row_updates_match_recomputation(n, p, truncated, asym, fp32):
             x[i] = x[i] + 0.1 * rng.standard_normal(p)
             e.updateLocations(i, x[i])
         s = e.getSumOfIncrements()
-        ref = _fresh_sum(n, p, truncated, fp32, delta, x)
+        ref = _oracle_sum(oracle, truncated, delta, x)
         assert abs(s - ref) <= tol * abs(ref), (step, s, ref)
     full1, rows1 = e.counters()
     assert rows1 - rows0 == 24 and full1 == full0  # every sum came from the O(N) path
     e.close()
 
 
-def test_state_rules():
+def test_state_rules(oracle):
     n, p = 200, 2
     delta = datagen.euclid10(n, 31, 0.1, 0.02)
     x = datagen.locations(n, p, 32)
@@ -75,7 +75,7 @@ def test_state_rules():
     e.updateLocations(9, x[9])
     s1 = e.getSumOfIncrements()
     assert e.counters() == (f + 1, r)
-    assert abs(s1 - _fresh_sum(n, p, True, False, delta, x)) <= 1e-12 * abs(s1)
+    assert abs(s1 - _oracle_sum(oracle, True, delta, x)) <= 1e-12 * abs(s1)
     # store, move one row, restore: the stored sum and locations come back
     e.storeState()
     x_old = x[17].copy()
@@ -91,14 +91,14 @@ def test_state_rules():
     e.updateLocations(3, x[3])
     s3 = e.getSumOfIncrements()
     assert e.counters() == (f + 1, r + 2)
-    assert abs(s3 - _fresh_sum(n, p, True, False, delta, x)) <= 1e-12 * abs(s3)
+    assert abs(s3 - _oracle_sum(oracle, True, delta, x)) <= 1e-12 * abs(s3)
     # new data: the cached sum stays (:257-260) but is no base for the O(N) path
     e.setPairwiseData(delta * 1.01)
     x[5] += 0.1
     e.updateLocations(5, x[5])
     s4 = e.getSumOfIncrements()
     assert e.counters() == (f + 2, r + 2)
-    assert abs(s4 - _fresh_sum(n, p, True, False, delta * 1.01, x)) <= 1e-12 * abs(s4)
+    assert abs(s4 - _oracle_sum(oracle, True, delta * 1.01, x)) <= 1e-12 * abs(s4)
     # a gradient call on a moved row computes everything (and the sum with it)
     x[6] += 0.1
     e.updateLocations(6, x[6])
@@ -109,7 +109,7 @@ def test_state_rules():
     e.close()
 
 
-def test_many_updates_stay_exact_and_refresh():
+def test_many_updates_stay_exact_and_refresh(oracle):
     """A long chain of accepted single-row moves (more than the limit of
     consecutive O(N) updates) stays within 1e-12 of a recomputation."""
     n, p = 400, 2
@@ -128,21 +128,18 @@ def test_many_updates_stay_exact_and_refresh():
     f1, r1 = e.counters()
     assert f1 - f0 == 1 and r1 - r0 == moves - 1  # one refresh after 1024 O(N) updates
     s = e.getSumOfIncrements()
-    ref = _fresh_sum(n, p, True, False, delta, x)
+    ref = _oracle_sum(oracle, True, delta, x)
     assert abs(s - ref) <= 1e-12 * abs(ref)
     e.close()
 
 
-def _fresh(n, p, delta, x, fp32=False):
-    e = _engine(n, p, True, fp32, delta, x)
-    s = e.getSumOfIncrements()
-    g = e.getLogLikelihoodGradient().reshape(n, p)
-    e.close()
-    return s, g
+def _fresh(oracle, delta, x, tau=1.7):
+    """Serial FP64 oracle: sum and gradient at locations x (truncated)."""
+    return oracle.sum_of_increments(delta, x, tau, True), oracle.gradient(delta, x, tau, True)
 
 
 @pytest.mark.parametrize("fp32", [False, True])
-def test_deferred_row_write_sequences(fp32):
+def test_deferred_row_write_sequences(oracle, fp32):
     """A single-row updateLocations leaves the device row at its previous
     value until the row kernel writes the new one; every other use of the
     locations writes it first.  Sequences that take those other routes
@@ -153,7 +150,7 @@ def test_deferred_row_write_sequences(fp32):
     tol = 2e-6 if fp32 else 1e-12
 
     def close(e, xs):
-        s_ref, g_ref = _fresh(n, p, delta, xs, fp32)
+        s_ref, g_ref = _fresh(oracle, delta, xs)
         s, g = e.getSumOfIncrements(), e.getLogLikelihoodGradient().reshape(n, p)
         assert abs(s - s_ref) <= tol * abs(s_ref), (s, s_ref)
         assert np.max(np.abs(g - g_ref)) <= (10 * tol if fp32 else tol) * np.max(np.abs(g_ref))
@@ -178,7 +175,7 @@ def test_deferred_row_write_sequences(fp32):
     e.updateLocations(1, x3[1])
     e.updateLocations(200, x3[200])
     g = e.getLogLikelihoodGradient().reshape(n, p)
-    g_ref = _fresh(n, p, delta, x3, fp32)[1]
+    g_ref = _fresh(oracle, delta, x3)[1]
     assert np.max(np.abs(g - g_ref)) <= (10 * tol if fp32 else tol) * np.max(np.abs(g_ref))
     close(e, x3)
     # store, move without asking for the sum, restore: the stored state comes back whole
@@ -196,4 +193,58 @@ def test_deferred_row_write_sequences(fp32):
         x3[42] += 0.1 * (step + 1)
         e.updateLocations(42, x3[42])
     close(e, x3)
+    e.close()
+
+
+def test_restore_does_not_reuse_a_stale_stored_sum(oracle):
+    """storeState keeps the sum as it is (the reference restores it as known
+    even when stale, src/NewMultiDimensionalScaling.hpp:205-236); a sum
+    stored before new observations, or stored while unknown, must not become
+    the base of the next single-row update, which the reference follows
+    with a full recomputation."""
+    n, p, tau = 200, 2, 1.7
+    d0 = datagen.euclid10(n, 61, 0.1, 0.03)
+    d1 = d0 * 1.02
+    x = datagen.locations(n, p, 62)
+    e = _engine(n, p, True, False, d0, x, tau)
+    e.getSumOfIncrements()          # known sum of (d0, x)
+    e.storeState()
+    e.setPairwiseData(d1)
+    x1 = x + 0.01
+    e.updateLocations(-1, x1)
+    e.getSumOfIncrements()          # full evaluation on d1
+    e.restoreState()                # locations x and the sum of (d0, x) come back
+    x2 = x.copy()
+    x2[7] += 0.3
+    e.updateLocations(7, x2[7])
+    s = e.getSumOfIncrements()
+    ref = oracle.sum_of_increments(d1, x2, tau, True)
+    assert abs(s - ref) <= 1e-12 * abs(ref), (s, ref)
+    # stored while unknown: the locations changed after the last sum
+    e.updateLocations(-1, x)
+    e.storeState()
+    e.updateLocations(-1, x1)
+    e.getSumOfIncrements()
+    e.restoreState()
+    x3 = x.copy()
+    x3[11] -= 0.2
+    e.updateLocations(11, x3[11])
+    s = e.getSumOfIncrements()
+    ref = oracle.sum_of_increments(d1, x3, tau, True)
+    assert abs(s - ref) <= 1e-12 * abs(ref), (s, ref)
+    # a valid stored sum still serves as the base of an O(N) update
+    e.storeState()
+    x4 = x3.copy()
+    x4[20] += 0.5
+    e.updateLocations(20, x4[20])
+    e.getSumOfIncrements()
+    e.restoreState()
+    f0, r0 = e.counters()
+    x5 = x3.copy()
+    x5[30] += 0.1
+    e.updateLocations(30, x5[30])
+    s = e.getSumOfIncrements()
+    assert e.counters() == (f0, r0 + 1)
+    ref = oracle.sum_of_increments(d1, x5, tau, True)
+    assert abs(s - ref) <= 1e-12 * abs(ref), (s, ref)
     e.close()
