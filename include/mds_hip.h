@@ -114,6 +114,34 @@ int mds_hip_get_internal_dimension(mds_hip_handle h, int* out);
 int mds_hip_evaluate_async(mds_hip_handle h, int computeGradient);
 int mds_hip_synchronize(mds_hip_handle h);
 
+/*
+ * Allocation-free evaluation.  No counterpart in the reference: this entry
+ * point is for callers that cannot allocate device memory while they
+ * evaluate (tight serving loops, harnesses that forbid hipMalloc).  It is
+ * mds_hip_evaluate_async with device-resident inputs and outputs, and it
+ * calls no hipMalloc, hipFree or allocating library routine: the tile sums
+ * and gradient partials go to a static device array of the library (64 MiB
+ * per GPU, held until the process ends) instead of the instance's buffers.
+ *   locations:   device, N x P row-major FP64; they become the instance's
+ *                current locations (as updateLocations(-1, ...)).  NULL
+ *                evaluates at the current locations.
+ *   sumOut:      device, one double: what getSumOfIncrements returns there.
+ *   gradientOut: device, N x P row-major FP64: the log-likelihood gradient;
+ *                NULL computes the likelihood alone.
+ * Each pointer must lie inside one device allocation of the instance's
+ * device, long enough, or the call fails with MDS_HIP_ERR_ARGUMENT.  So does
+ * an instance whose gradient partials exceed the array (8 * tiles + 512 *
+ * P' * ceil(N / 64)^2 bytes, P' the kernel dimension >= P: with the gradient,
+ * N <= 16,320 at P <= 2).  The work runs on the instance's stream, after
+ * the instance's earlier work, and the call returns when the outputs are
+ * written; the inputs must be ready when it is made.  Calls on one GPU share
+ * the array and it keeps the last call's contents, so calls that overlap in
+ * time (instances used from different threads) race and must be serialised
+ * by the caller; calls made one after another never overlap.
+ */
+int mds_hip_evaluate_device_noalloc(mds_hip_handle h, const double* locations, double* sumOut,
+                                    double* gradientOut);
+
 /* Kernel timing with HIP events on the instance stream.  While enabled,
  * every evaluation records the tile-kernel and whole-evaluation durations;
  * get_timing synchronises and returns their sums (ms) and the count. */

@@ -37,7 +37,7 @@ HIP_FLAGS = [
 
 # The library: every .hip file under csrc/ except stand-alone programs.
 LIB_SOURCES = ["mds_kernels.hip", "mds_row.hip", "mds_engine.hip", "mds_batch.hip", "mds_hmc.hip", "mds_mlp.hip",
-               "mds_singleton.hip"]
+               "mds_singleton.hip", "mds_noalloc.hip"]
 
 
 def _run(cmd: list[str]) -> None:

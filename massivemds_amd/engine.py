@@ -75,6 +75,7 @@ def load_library() -> ctypes.CDLL:
         "mds_hip_get_internal_dimension": ([H, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "mds_hip_evaluate_async": ([H, ctypes.c_int], ctypes.c_int),
         "mds_hip_synchronize": ([H], ctypes.c_int),
+        "mds_hip_evaluate_device_noalloc": ([H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
         "mds_hip_set_timing": ([H, ctypes.c_int], ctypes.c_int),
         "mds_hip_get_timing": ([H, _c_double_p, _c_double_p, ctypes.POINTER(ctypes.c_longlong)],
                                ctypes.c_int),
@@ -190,6 +191,12 @@ class MultiDimensionalScaling:
 
     def synchronize(self) -> None:
         self._check(self._lib.mds_hip_synchronize(self._h))
+
+    def evaluate_device_noalloc(self, locations_ptr: int | None, sum_ptr: int, gradient_ptr: int | None) -> None:
+        """mds_hip_evaluate_device_noalloc: device addresses of the N x P FP64
+        locations (None: the current ones), of one FP64 for the sum and of
+        the N x P FP64 gradient (None: likelihood only)."""
+        self._check(self._lib.mds_hip_evaluate_device_noalloc(self._h, locations_ptr, sum_ptr, gradient_ptr))
 
     def set_timing(self, enable: bool) -> None:
         self._check(self._lib.mds_hip_set_timing(self._h, 1 if enable else 0))
