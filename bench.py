@@ -36,15 +36,17 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
+FP32_VALU_PEAK_TFLOPS = 157.3  # packed FP32; one FP32 FMA per lane per cycle reaches half of it
 
 
-def _profile_figure(key):
-    """A per-launch figure of the FP64 tile kernel at this workload from the
-    rocprofv3 runs summarised in profiles/tile_kernel_n10000_p2.md; None when
-    the summary is absent."""
+def _profile_figure(key, summary="tile_kernel_n10000_p2.md"):
+    """A per-launch figure from a rocprofv3 summary under profiles/ (by
+    default the FP64 tile kernel at the metric's workload,
+    profiles/tile_kernel_n10000_p2.md): the number on the line
+    "<key>: <value>"; None when absent."""
     from pathlib import Path
 
-    path = Path(__file__).resolve().parent / "profiles" / "tile_kernel_n10000_p2.md"
+    path = Path(__file__).resolve().parent / "profiles" / summary
     try:
         for line in path.read_text().splitlines():
             if line.startswith(key + ":"):
@@ -75,6 +77,19 @@ def _valu_roofline(tile_s):
     return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_WAVE_INSTR_PER_S,
             "unit": "wave instructions/s", "frac": achieved / VALU_PEAK_WAVE_INSTR_PER_S,
             "instructions": instr}
+
+
+def _flop_roofline(key, seconds, peak_tflops):
+    """A tile kernel against the MI355X vector FLOP peak: FLOPs per launch
+    (rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 or _F32, 64 lanes each,
+    an FMA counting 2, from the profile summary) over the measured
+    duration."""
+    flops = _profile_figure(key)
+    if flops is None:
+        return None
+    achieved = flops / seconds / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": peak_tflops, "unit": "TFLOP/s",
+            "frac": achieved / peak_tflops, "flops": flops}
 
 
 def _workload(n=10000):
@@ -233,11 +248,14 @@ def _cpu_baseline(n, p, tau, delta, x, budget_s=20.0):
     }
 
 
-def _bench_batch(steps, warmup, count=64, tune_cache=None):
+def _bench_batch(steps, warmup, count=64, tune_cache=None, no_cpu=False, cpu_budget=10.0):
     """Uneven batch: `count` datasets, N log-uniform in [500, 8000], P in
     {2, 5, 10}, tau in [0.5, 4], truncation, FP32 with FP64 reductions; one
     persistent launch per evaluation against `count` sequential
-    single-instance evaluations of the same data."""
+    single-instance evaluations of the same data.  The CPU figure times the
+    all-core SIMD port (likelihood pass + gradient pass, FP64) on the
+    smallest datasets within the budget and scales its pair rate to the
+    whole batch."""
     from massivemds_amd import datagen
     from massivemds_amd.batch import MultiDimensionalScalingBatch
     from massivemds_amd.engine import Flags, MultiDimensionalScaling
@@ -290,10 +308,35 @@ def _bench_batch(steps, warmup, count=64, tune_cache=None):
         "config": dict(zip(["wavesPerBlock", "blocksPerCU", "tilesPerItem", "strategy"], b.get_config())),
         "autotune_s": tune_s, "autotune_best_ms": tuned_ms, "bitwise_equal_to_sequential": identical,
     }
+    achieved = algo / (ev_ms / cnt * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": _profile_figure(
+                           "traffic_bytes_batch", "batch64_uneven_fp32.md")}
     for e in singles:
         e.close()
     b.close()
-    print(json.dumps(out))
+    if not no_cpu:
+        from oracle import oracle
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or oracle.max_threads()
+        spent, done_pairs, used = 0.0, 0, 0
+        for n, p, tau, seed in sorted(specs):
+            if spent > cpu_budget and used:
+                break
+            delta, x = datagen.batch_dataset(n, p, seed)
+            t0 = time.perf_counter()
+            oracle.simd_sum_of_increments(delta, x, tau, True, threads=threads)
+            oracle.simd_gradient(delta, x, tau, True, threads=threads)
+            spent += time.perf_counter() - t0
+            done_pairs += n * (n - 1) // 2
+            used += 1
+        rate = done_pairs / spent / pairs  # batch evaluations per second
+        out["cpu_baseline"] = {"value": rate, "unit": "batch evals/s", "cores": threads, "kind": "port",
+                               "sample": f"the {used} smallest datasets (likelihood pass + gradient pass, FP64, "
+                                         f"SIMD + OpenMP, {threads} threads), pair rate scaled to the batch's "
+                                         f"{pairs} pairs"}
+        out["gpu_vs_cpu"] = {"speedup": out["batch_evals_per_s"] / rate}
+    return out
 
 
 def _bench_hmc(iterations=2000, burn_in=500, n=5000, p=2):
@@ -502,6 +545,13 @@ def _bench_reference_loop(iterations=100, n=1000, p=2, tau=1.0, no_cpu=False, cp
         "api_us_per_iteration": api_s / iterations * 1e6,
         "resident_evals_per_s": 1e3 / resident["wall_ms_per_step"],
         "tile_kernel_us": resident["tile_ms"] * 1e3,
+        # 4.0 MB per evaluation, resident in the 256 MB Infinity Cache after
+        # the first pass: an effective rate against the HBM peak, not an HBM
+        # measurement
+        "roofline": {"bound": "hbm", "achieved": (8 * n * (n - 1) // 2 + 16 * n * p) / (resident["tile_ms"] * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s (effective, cache-resident)",
+                     "frac": (8 * n * (n - 1) // 2 + 16 * n * p) / (resident["tile_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None},
         "data": "synthetic (euclid10 distances, no noise, no missing; fresh N(0, I_2) locations per iteration)",
     }
     if not no_cpu:
@@ -535,7 +585,7 @@ def _bench_reference_loop(iterations=100, n=1000, p=2, tau=1.0, no_cpu=False, cp
                              "api_speedup_vs_serial": out["api_evals_per_s"] * float(np.median(ser)),
                              "max_rel_err_sum": max(errs_s), "max_rel_err_grad": max(errs_g),
                              "iterations_checked": k}
-    print(json.dumps(out))
+    return out
 
 
 def _calibrate_copy(mib):
@@ -563,6 +613,9 @@ def main():
                     help="untimed evaluations after the warm-up for this many ms of wall time (the GPU clock "
                          "settles under sustained load), before each timed pass of the default and leapfrog5000 "
                          "workloads; 0: none")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default workload only: skip the reference loop at N = 1000 and the uneven batch that "
+                         "the line carries beside it")
     ap.add_argument("--n", type=int, default=10000,
                     help="locations for the default workload (the metric is quoted at 10000; other sizes "
                          "for scaling studies)")
@@ -589,10 +642,10 @@ def main():
         _calibrate_copy(args.calibrate_copy)
         return
     if args.workload == "n1000":
-        _bench_reference_loop(no_cpu=args.no_cpu)
+        print(json.dumps(_bench_reference_loop(no_cpu=args.no_cpu)))
         return
     if args.workload == "batch64":
-        _bench_batch(args.steps, args.warmup, tune_cache=args.tune_cache)
+        print(json.dumps(_bench_batch(args.steps, args.warmup, tune_cache=args.tune_cache, no_cpu=args.no_cpu)))
         return
     if args.workload == "hmc5000":
         _bench_hmc()
@@ -649,11 +702,15 @@ def main():
             "traffic": _measured_traffic() if n == 10000 else None,  # profiled at N = 10000
         },
         "roofline_valu": _valu_roofline(r64["tile_ms"] * 1e-3) if n == 10000 else None,
+        "roofline_fp64": (_flop_roofline("fp64_flops_fp64_tile", r64["tile_ms"] * 1e-3, FP64_VALU_PEAK_TFLOPS)
+                          if n == 10000 else None),
         "fp32": {
             "value": 1e3 / r32["wall_ms_per_step"],
             "ms_per_step": r32["wall_ms_per_step"],
             "tile_kernel_ms": r32["tile_ms"],
             "achieved_GBs": bytes32 / (r32["tile_ms"] * 1e-3) / 1e9,
+            "roofline_fp32": (_flop_roofline("fp32_flops_fp32_tile", r32["tile_ms"] * 1e-3, FP32_VALU_PEAK_TFLOPS)
+                              if n == 10000 else None),
             "rel_err_loglik_vs_fp64": abs(ll32 - ll64) / abs(ll64),
             "rel_err_grad_vs_fp64": float(np.max(np.abs(r32["grad"] - r64["grad"])) / np.max(np.abs(r64["grad"]))),
         },
@@ -672,6 +729,14 @@ def main():
                              "rel_err_grad": rel_g,
                              "speedup_vs_serial_1core": out["value"] / cpu["variants"]["serial_1core"]["value"],
                              "speedup_vs_simd_1core": out["value"] / cpu["variants"]["simd_1core"]["value"]}
+    if not args.no_extra and n == 10000:
+        # the other single-GPU configurations of the metric's family, each
+        # with its own rate, roofline figures and CPU comparison
+        out["other_configs"] = {
+            "reference_loop_n1000": _bench_reference_loop(no_cpu=args.no_cpu, cpu_budget=min(args.cpu_budget, 10.0)),
+            "batch64_uneven_fp32": _bench_batch(20, 3, no_cpu=args.no_cpu,
+                                                cpu_budget=min(args.cpu_budget, 10.0)),
+        }
     print(json.dumps(out))
 
 

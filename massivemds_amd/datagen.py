@@ -79,6 +79,47 @@ def _upper_pairs_mask(n: int, frac: float, seed: int, key: int) -> np.ndarray:
     return m | m.T
 
 
+def _distances(z: np.ndarray, block: int) -> np.ndarray:
+    """||z_i - z_j||_2 for all i, j, a block of rows at a time into buffers
+    allocated once.  The squared coordinate differences are added in the
+    order numpy's pairwise summation uses for a row of ``dim`` values
+    (``np.sum(d * d, axis=-1)``: eight partial sums combined as a tree, the
+    rest added in turn), so the values are those of that expression bit for
+    bit."""
+    n, dim = z.shape
+    zt = np.ascontiguousarray(z.T)
+    delta = np.empty((n, n))
+    rows = min(block, n)
+    sq = np.empty((dim, rows, n))
+    a = np.empty((rows, n))
+    b = np.empty((rows, n))
+    for r0 in range(0, n, block):
+        r1 = min(n, r0 + block)
+        m = r1 - r0
+        q = sq[:, :m]
+        for k in range(dim):
+            np.subtract(zt[k, r0:r1, None], zt[k, None, :], out=q[k])
+            np.multiply(q[k], q[k], out=q[k])
+        out = delta[r0:r1]
+        if dim >= 8:
+            am, bm = a[:m], b[:m]
+            np.add(q[0], q[1], out=am)
+            np.add(q[2], q[3], out=bm)
+            np.add(am, bm, out=am)
+            np.add(q[4], q[5], out=bm)
+            np.add(q[6], q[7], out=out)
+            np.add(bm, out, out=bm)
+            np.add(am, bm, out=out)
+            for k in range(8, dim):
+                np.add(out, q[k], out=out)
+        else:
+            np.copyto(out, q[0])
+            for k in range(1, dim):
+                np.add(out, q[k], out=out)
+        np.sqrt(out, out=out)
+    return delta
+
+
 def euclid10(n: int, seed: int = DATA_SEED, noise_sd: float = 0.0, missing: float = 0.0,
              dim: int = 10, block: int = 512) -> np.ndarray:
     """Row-major N x N observations: distances of N(0, I_dim) points.
@@ -88,11 +129,7 @@ def euclid10(n: int, seed: int = DATA_SEED, noise_sd: float = 0.0, missing: floa
     pairs set to NaN on both sides.
     """
     z = normals(seed, 1, n * dim).reshape(n, dim)
-    delta = np.empty((n, n))
-    for r0 in range(0, n, block):
-        r1 = min(n, r0 + block)
-        diff = z[r0:r1, None, :] - z[None, :, :]
-        delta[r0:r1] = np.sqrt(np.sum(diff * diff, axis=2))
+    delta = _distances(z, block)
     np.fill_diagonal(delta, 0.0)
     if noise_sd > 0.0:
         iu = np.triu_indices(n, 1)
