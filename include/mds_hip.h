@@ -130,9 +130,9 @@ int mds_hip_synchronize(mds_hip_handle h);
  *                NULL computes the likelihood alone.
  * Each pointer must lie inside one device allocation of the instance's
  * device, long enough, or the call fails with MDS_HIP_ERR_ARGUMENT.  So does
- * an instance whose gradient partials exceed the array (8 * tiles + 512 *
- * P' * ceil(N / 64)^2 bytes, P' the kernel dimension >= P: with the gradient,
- * N <= 16,320 at P <= 2).  The work runs on the instance's stream, after
+ * an instance whose gradient partials exceed the array (8 * tiles + 64 * s *
+ * P' * ceil(N / 64)^2 bytes, s = 8 for the FP64 engine and 4 for FP32, P' the
+ * kernel dimension >= P: with the gradient, N <= 16,320 at P <= 2 in FP64).  The work runs on the instance's stream, after
  * the instance's earlier work, and the call returns when the outputs are
  * written; the inputs must be ready when it is made.  Calls on one GPU share
  * the array and it keeps the last call's contents, so calls that overlap in
@@ -189,6 +189,11 @@ int mds_hip_batch_set_config(mds_hip_batch_handle h, int wavesPerBlock, int bloc
                              int strategy);
 int mds_hip_batch_get_config(mds_hip_batch_handle h, int* wavesPerBlock, int* blocksPerCU, int* tilesPerItem,
                              int* strategy);
+/* Entry `index` reads the observations of entry `sourceIndex` (same N; P may
+ * differ) instead of its own: several chains of one dataset without copies.
+ * Later data set on the source are seen by `index`; data set on `index`
+ * itself end the sharing. */
+int mds_hip_batch_share_pairwise_data(mds_hip_batch_handle h, int index, int sourceIndex);
 int mds_hip_batch_set_timing(mds_hip_batch_handle h, int enable);
 int mds_hip_batch_get_timing(mds_hip_batch_handle h, double* ms, long long* count);
 
@@ -275,6 +280,10 @@ typedef struct {
     int fallbackWindow;          /* iterations, exact gradients take over for fallbackWindow
                                   * iterations, doubled after each consecutive failed window
                                   * (at most 1024x); 0 disables the fallback */
+    int adaptPrecisionWidth;     /* truncation: during burn-in, adapt the width w of the
+                                  * precision's proposal |U(tau - w, tau + w)| toward 0.44
+                                  * acceptance (Robbins-Monro on log w); fixed afterwards.
+                                  * 0 keeps precisionProposalWidth, as the reference does */
 } mds_hip_hmc_options;
 
 typedef struct {
@@ -290,12 +299,33 @@ typedef struct {
     double exactIterations;
     double snapshots;            /* snapshots stored in this run */
     double surrogateAcceptRate;  /* acceptance over the surrogate iterations alone */
+    double precisionProposalWidth;  /* the precision proposal's width after burn-in */
 } mds_hip_hmc_stats;
 
 int mds_hip_hmc_default_options(mds_hip_hmc_options* options);
 int mds_hip_hmc_run(mds_hip_handle h, const mds_hip_hmc_options* options, const double* initialLocations,
                     size_t length, double* trace, size_t traceLength, double* finalLocations,
                     mds_hip_hmc_stats* stats);
+
+/*
+ * The sampler above on every entry of a batch at once (no counterpart in the
+ * reference; many chains, of one dataset through
+ * mds_hip_batch_share_pairwise_data or of datasets of uneven N and P): one
+ * batched tile launch, one batched reduce and one kick launch per leapfrog
+ * step move every chain, the trajectory captured as one hipGraph.  Entry k
+ * runs its own chain with seed options->seed + k, its own precision (from
+ * its setParameters), step size and trait precision, and gives the bits of
+ * mds_hip_hmc_run on a single engine with that seed and the same data.
+ * Exact gradients only (useSurrogate and collectSnapshots must be 0).
+ * initialLocations / finalLocations: the entries' N_k x P_k arrays
+ * concatenated (length sum N_k P_k; finalLocations may be NULL); traces:
+ * count x (iterations - burnIn) x MDS_HIP_HMC_TRACE_COLUMNS (or NULL);
+ * stats: statsCount = count entries (or NULL).  On return each entry holds
+ * its chain's final locations and precision.
+ */
+int mds_hip_batch_hmc_run(mds_hip_batch_handle h, const mds_hip_hmc_options* options, const double* initialLocations,
+                          size_t length, double* traces, size_t traceLength, double* finalLocations,
+                          size_t finalLength, mds_hip_hmc_stats* stats, int statsCount);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,10 @@ in one persistent launch (C ABI: mds_hip_batch_* in include/mds_hip.h).
 Each dataset keeps the single-instance semantics: its own observations,
 locations and precision; `sums()` returns every dataset's
 getSumOfIncrements() and `gradient(k)` the N_k x P_k log-likelihood gradient.
+Entries can share another entry's observations (`sharePairwiseData`), so
+that several chains of one dataset run side by side, and `run_hmc` runs the
+device HMC sampler on every entry at once, one launch per leapfrog step for
+all chains.
 """
 from __future__ import annotations
 
@@ -38,6 +42,9 @@ def _bind(lib):
         "mds_hip_batch_get_config": ([H, ip, ip, ip, ip], ctypes.c_int),
         "mds_hip_batch_set_timing": ([H, ctypes.c_int], ctypes.c_int),
         "mds_hip_batch_get_timing": ([H, _c_double_p, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+        "mds_hip_batch_share_pairwise_data": ([H, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "mds_hip_batch_hmc_run": ([H, ctypes.c_void_p, _c_double_p, ctypes.c_size_t, _c_double_p, ctypes.c_size_t,
+                                   _c_double_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -126,3 +133,32 @@ class MultiDimensionalScalingBatch:
         ms, c = ctypes.c_double(), ctypes.c_longlong()
         self._check(self._lib.mds_hip_batch_get_timing(self._h, ctypes.byref(ms), ctypes.byref(c)))
         return ms.value, c.value
+
+    def sharePairwiseData(self, index: int, source: int) -> None:
+        """Entry `index` reads entry `source`'s observations (same N)."""
+        self._check(self._lib.mds_hip_batch_share_pairwise_data(self._h, int(index), int(source)))
+
+    def run_hmc(self, initials, options):
+        """The device HMC sampler on every entry at once (one chain per
+        entry; chain k uses seed options.seed + k).  `initials`: one N_k x P_k
+        array per entry.  Returns (traces, final locations, stats), one per
+        entry, with the single-chain sampler's trace columns."""
+        from .hmc import TRACE_COLUMNS, HmcStats
+
+        if len(initials) != len(self.ns):
+            raise ValueError("one initial location array per entry")
+        x0 = np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in initials])
+        kept = options.iterations - options.burnIn
+        count = len(self.ns)
+        traces = np.empty(count * kept * len(TRACE_COLUMNS))
+        finals = np.empty(x0.size)
+        stats = (HmcStats * count)()
+        self._check(self._lib.mds_hip_batch_hmc_run(self._h, ctypes.addressof(options), _ptr(x0), x0.size,
+                                                    _ptr(traces), traces.size, _ptr(finals), finals.size,
+                                                    ctypes.addressof(stats), count))
+        traces = traces.reshape(count, kept, len(TRACE_COLUMNS))
+        out, k = [], 0
+        for n, p in zip(self.ns, self.dims):
+            out.append(finals[k:k + n * p].reshape(n, p))
+            k += n * p
+        return list(traces), out, list(stats)

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -30,6 +31,8 @@
 #include <vector>
 
 #include "mds_hip.h"
+#include "mds_hmc.hpp"
+#include "mds_hmc_device.hpp"
 #include "mds_host.hpp"
 #include "mds_internal.hpp"
 #include "mds_tile.hpp"
@@ -41,10 +44,11 @@ struct BatchDataset {
     const void* delta;      // packed tiles (Real), planes as in the engine
     const void* x;          // [nb * 64][kdim] Real
     const uint32_t* tiles;  // tile codes (I << 16) | J
-    double* part;           // [nb][nb][kdim][64]
+    void* part;             // [nb][nb][kdim][64], Real
     double* grad;           // [n][p]
     int64_t planeStride;
     double tau;
+    const double* devTau;   // if set, the precision is read here (a sampler's parameter block)
     int nb, n, p, kdim;
     int truncated, symmetric;
     int tileSumBase, numTiles;
@@ -71,25 +75,26 @@ __device__ __forceinline__ void batch_tile(const BatchDataset& d, uint32_t t, do
     const int I = (int)(code >> 16), J = (int)(code & 0xffffu);
     const Real* base = (const Real*)d.delta + (int64_t)t * kTileElems;
     const Real* x = (const Real*)d.x;
-    const PairConst<Real> pc = make_pair_const<Real>(d.tau, lnPhiTab);
+    const PairConst<Real> pc = make_pair_const<Real>(d.devTau ? *d.devTau : d.tau, lnPhiTab);
     double* sum = tileSums + d.tileSumBase + t;
+    Real* part = (Real*)d.part;
     // the previous tile's reads of the staged columns precede the new writes
     wave_lds_sync();
     stage_columns<Real, P>(stage, x, J);
     wave_lds_sync();
     if (d.truncated) {
         if (d.symmetric)
-            tile_walk<Real, P, true, true, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum, d.part);
+            tile_walk<Real, P, true, true, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum, part);
         else
             tile_walk<Real, P, true, false, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum,
-                                                          d.part);
+                                                          part);
     } else {
         if (d.symmetric)
             tile_walk<Real, P, false, true, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum,
-                                                          d.part);
+                                                          part);
         else
             tile_walk<Real, P, false, false, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum,
-                                                           d.part);
+                                                           part);
     }
 }
 
@@ -144,7 +149,7 @@ constexpr int kBatchReduceWaves = 8;
 // wavefront partials are combined per coordinate in wavefront order.  Per
 // coordinate the additions are those of the single-instance reduce, in the
 // same order, so the rows are bitwise the same.
-template <int KD>
+template <typename PartT, int KD>
 __device__ __forceinline__ void batch_reduce_rows(const BatchDataset& d, int K, double* red) {
     constexpr int kAhead = 4;
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
@@ -152,15 +157,15 @@ __device__ __forceinline__ void batch_reduce_rows(const BatchDataset& d, int K, 
 #pragma unroll
     for (int p = 0; p < KD; ++p) acc[p] = 0.0;
     const int64_t stride = (int64_t)KD * kTile;  // between column blocks J
-    const double* base = d.part + (int64_t)K * d.nb * stride + lane;
+    const PartT* base = (const PartT*)d.part + (int64_t)K * d.nb * stride + lane;
     int J = wave;
     for (; J + (kAhead - 1) * kBatchReduceWaves < d.nb; J += kAhead * kBatchReduceWaves) {
         double v[kAhead][KD];
 #pragma unroll
         for (int u = 0; u < kAhead; ++u) {
-            const double* slot = base + (J + u * kBatchReduceWaves) * stride;
+            const PartT* slot = base + (J + u * kBatchReduceWaves) * stride;
 #pragma unroll
-            for (int p = 0; p < KD; ++p) v[u][p] = slot[p * kTile];
+            for (int p = 0; p < KD; ++p) v[u][p] = (double)slot[p * kTile];
         }
 #pragma unroll
         for (int u = 0; u < kAhead; ++u)
@@ -168,9 +173,9 @@ __device__ __forceinline__ void batch_reduce_rows(const BatchDataset& d, int K, 
             for (int p = 0; p < KD; ++p) acc[p] += v[u][p];
     }
     for (; J < d.nb; J += kBatchReduceWaves) {
-        const double* slot = base + J * stride;
+        const PartT* slot = base + J * stride;
 #pragma unroll
-        for (int p = 0; p < KD; ++p) acc[p] += slot[p * kTile];
+        for (int p = 0; p < KD; ++p) acc[p] += (double)slot[p * kTile];
     }
     const int row = K * kTile + lane;
 #pragma unroll
@@ -186,6 +191,7 @@ __device__ __forceinline__ void batch_reduce_rows(const BatchDataset& d, int K, 
     }
 }
 
+template <typename PartT>
 __global__ __launch_bounds__(kBatchReduceWaves * kWave) void batch_reduce_kernel(
     const BatchDataset* __restrict__ ds, const ReduceItem* __restrict__ items,
     const double* __restrict__ tileSums, double* __restrict__ sums) {
@@ -218,14 +224,14 @@ __global__ __launch_bounds__(kBatchReduceWaves * kWave) void batch_reduce_kernel
         return;
     }
     switch (d.kdim) {  // uniform over the block
-        case 1: batch_reduce_rows<1>(d, item.K, red); break;
-        case 2: batch_reduce_rows<2>(d, item.K, red); break;
-        case 3: batch_reduce_rows<3>(d, item.K, red); break;
-        case 4: batch_reduce_rows<4>(d, item.K, red); break;
-        case 5: batch_reduce_rows<5>(d, item.K, red); break;
-        case 6: batch_reduce_rows<6>(d, item.K, red); break;
-        case 8: batch_reduce_rows<8>(d, item.K, red); break;
-        case 10: batch_reduce_rows<10>(d, item.K, red); break;
+        case 1: batch_reduce_rows<PartT, 1>(d, item.K, red); break;
+        case 2: batch_reduce_rows<PartT, 2>(d, item.K, red); break;
+        case 3: batch_reduce_rows<PartT, 3>(d, item.K, red); break;
+        case 4: batch_reduce_rows<PartT, 4>(d, item.K, red); break;
+        case 5: batch_reduce_rows<PartT, 5>(d, item.K, red); break;
+        case 6: batch_reduce_rows<PartT, 6>(d, item.K, red); break;
+        case 8: batch_reduce_rows<PartT, 8>(d, item.K, red); break;
+        case 10: batch_reduce_rows<PartT, 10>(d, item.K, red); break;
         default: break;  // the host only admits kernel dimensions <= 10 here
     }
 }
@@ -243,10 +249,12 @@ struct BatchEntry {
     int P = 0, N = 0, kdim = 0, nb = 0, numTiles = 0;
     bool symmetric = true, dataSet = false, precisionSet = false;
     double tau = 0.0;
-    void* dDelta = nullptr;
+    const double* devTau = nullptr;  // set by a sampler for the length of its run
+    void* dDelta = nullptr;          // owned packed observations (null while shared)
     size_t deltaBytes = 0;
+    int shareSource = -1;            // >= 0: reads that entry's packed observations
     void* dX = nullptr;
-    double* dPart = nullptr;
+    void* dPart = nullptr;  // gradient partials in the batch's real type
     double* dGrad = nullptr;
     uint32_t* dTiles = nullptr;
     std::vector<uint32_t> tiles;
@@ -328,8 +336,7 @@ struct Batch {
                 const size_t xBytes = (size_t)b.nb * kTile * b.kdim * realSize;
                 check(hipMalloc(&b.dX, xBytes), "hipMalloc(locations)");
                 check(hipMemsetAsync(b.dX, 0, xBytes, stream), "hipMemset(locations)");
-                check(hipMalloc((void**)&b.dPart, (size_t)b.nb * b.nb * b.kdim * kTile * sizeof(double)),
-                      "hipMalloc(partials)");
+                check(hipMalloc(&b.dPart, (size_t)b.nb * b.nb * b.kdim * kTile * realSize), "hipMalloc(partials)");
                 check(hipMalloc((void**)&b.dGrad, (size_t)b.N * b.P * sizeof(double)), "hipMalloc(gradient)");
                 check(hipMalloc((void**)&b.dTiles, b.tiles.size() * sizeof(uint32_t)), "hipMalloc(tiles)");
                 check(hipMemcpyAsync(b.dTiles, b.tiles.data(), b.tiles.size() * sizeof(uint32_t),
@@ -360,8 +367,8 @@ struct Batch {
             (void)hipFree(b.dPart);
             (void)hipFree(b.dGrad);
             (void)hipFree(b.dTiles);
-            b.dDelta = b.dX = nullptr;
-            b.dPart = b.dGrad = nullptr;
+            b.dDelta = b.dX = b.dPart = nullptr;
+            b.dGrad = nullptr;
             b.dTiles = nullptr;
         }
         (void)hipFree(dTileSums);
@@ -394,10 +401,33 @@ struct Batch {
         return e[index];
     }
 
+    // Entry `index` reads the packed observations of entry `source` (same
+    // N; P may differ), e.g. several chains of one dataset: no copy.  Later
+    // data for `source` are seen by `index` too; data set on `index` itself
+    // end the sharing.
+    void sharePairwiseData(int index, int source) {
+        BatchEntry& b = at(index);
+        const BatchEntry& src = at(source);
+        if (index == source) throw Error(MDS_HIP_ERR_ARGUMENT, "an entry cannot share its own data");
+        if (src.shareSource >= 0) throw Error(MDS_HIP_ERR_ARGUMENT, "the source entry shares another's data");
+        if (!src.dataSet) throw Error(MDS_HIP_ERR_STATE, "the source entry has no data");
+        if (src.N != b.N) throw Error(MDS_HIP_ERR_ARGUMENT, "shared data need the same location count");
+        for (const BatchEntry& o : e)
+            if (o.shareSource == index) throw Error(MDS_HIP_ERR_ARGUMENT, "other entries share this entry's data");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        check(hipFree(b.dDelta), "hipFree");
+        b.dDelta = nullptr;
+        b.deltaBytes = 0;
+        b.shareSource = source;
+        b.dataSet = true;
+        planDirty = true;
+    }
+
     void setPairwiseData(int index, const double* data, size_t length) {
         BatchEntry& b = at(index);
         if (!data || length != (size_t)b.N * b.N)
             throw Error(MDS_HIP_ERR_ARGUMENT, "setPairwiseData: length must be N*N");
+        b.shareSource = -1;
         const size_t planeElems = (size_t)b.numTiles * kTileElems;
         std::unique_ptr<char[]> host(new char[2 * planeElems * realSize]);
         const bool sym = fp32 ? pack_tiles<float>(data, b.N, b.tiles, (float*)host.get(),
@@ -465,20 +495,22 @@ struct Batch {
         std::vector<BatchDataset> desc(e.size());
         for (size_t k = 0; k < e.size(); ++k) {
             const BatchEntry& b = e[k];
+            const BatchEntry& data = b.shareSource >= 0 ? e[b.shareSource] : b;
             BatchDataset& d = desc[k];
-            d.delta = b.dDelta;
+            d.delta = data.dDelta;
             d.x = b.dX;
             d.tiles = b.dTiles;
             d.part = b.dPart;
             d.grad = b.dGrad;
             d.planeStride = (int64_t)b.numTiles * kTileElems;
             d.tau = b.tau;
+            d.devTau = b.devTau;
             d.nb = b.nb;
             d.n = b.N;
             d.p = b.P;
             d.kdim = b.kdim;
             d.truncated = truncated ? 1 : 0;
-            d.symmetric = b.symmetric ? 1 : 0;
+            d.symmetric = data.symmetric ? 1 : 0;
             d.tileSumBase = b.tileSumBase;
             d.numTiles = b.numTiles;
         }
@@ -575,8 +607,12 @@ struct Batch {
             check(hipEventRecord(joinEvent[k], side[k]), "hipEventRecord(join)");
             check(hipStreamWaitEvent(stream, joinEvent[k], 0), "hipStreamWaitEvent(join)");
         }
-        hipLaunchKernelGGL(batch_reduce_kernel, dim3(numReduce), dim3(kBatchReduceWaves * kWave), 0, stream, dDesc,
-                           dReduce, dTileSums, dSums);
+        if (fp32)
+            hipLaunchKernelGGL(batch_reduce_kernel<float>, dim3(numReduce), dim3(kBatchReduceWaves * kWave), 0, stream,
+                               dDesc, dReduce, dTileSums, dSums);
+        else
+            hipLaunchKernelGGL(batch_reduce_kernel<double>, dim3(numReduce), dim3(kBatchReduceWaves * kWave), 0,
+                               stream, dDesc, dReduce, dTileSums, dSums);
         check(hipGetLastError(), "batch reduce kernel");
     }
 
@@ -677,6 +713,566 @@ struct Batch {
         pending.clear();
         if (ms) *ms = timedMs;
         if (count) *count = timedCount;
+    }
+};
+
+// ---------------------------------------------------------------- sampler
+//
+// Many chains in one launch per leapfrog step: the HMC sampler of
+// mds_hmc.hip (R/Untitled.R:477-729, no tree) run over every entry of a
+// batch at once, one chain per entry -- several chains of one dataset
+// (entries sharing its observations) or chains of different datasets of
+// their own N and P.  Per leapfrog step one batched tile launch and one
+// batched reduce evaluate every chain's likelihood and gradient and one
+// kick launch moves every chain; the trajectory of L steps is captured once
+// as a hipGraph and replayed per iteration.  Each chain keeps its own
+// precision, step size and trait precision in a device parameter block
+// (the tile kernel reads its precision there), its own random streams
+// (seed + chain index, as a single-chain run with that seed) and its own
+// Metropolis tests in FP64 on the host, in the single-chain sampler's
+// order, so chain k gives the bits of mds_hip_hmc_run with seed + k.
+
+struct ChainDesc {
+    void* x;                // the entry's locations, Real [nb * 64][kdim]
+    double* mom;            // [n][p]
+    const double* grad;     // the entry's gradient [n][p]
+    const double* params;   // kHmcParamCount values
+    double* energy;         // [2]: kinetic energy, x^T T x sum
+    double* rows;           // [kTraceRows][p]
+    double* gram;           // [p][p]
+    TraceRows traceRows;
+    uint64_t seed;
+    int n, p, kdim;
+};
+
+constexpr int kChainThreads = 256;
+
+__global__ __launch_bounds__(kChainThreads) void chains_momentum_kernel(const ChainDesc* __restrict__ cd,
+                                                                        uint64_t iteration) {
+    const ChainDesc& d = cd[blockIdx.y];
+    momentum_pair(d.mom, d.n * d.p, d.seed, iteration, blockIdx.x * kChainThreads + threadIdx.x);
+}
+
+template <typename Real, int FORM>
+__global__ __launch_bounds__(kChainThreads) void chains_kick_kernel(const ChainDesc* __restrict__ cd) {
+    const ChainDesc& d = cd[blockIdx.y];
+    kick_row<Real, FORM>((Real*)d.x, d.mom, d.grad, d.params, d.n, d.p, d.kdim, blockIdx.x * kChainThreads + threadIdx.x);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kEnergyThreads) void chains_energy_kernel(const ChainDesc* __restrict__ cd) {
+    __shared__ double red[2][kEnergyThreads];
+    const ChainDesc& d = cd[blockIdx.x];
+    energy_block<Real>((const Real*)d.x, d.mom, d.params, d.n, d.p, d.kdim, d.energy, red);
+}
+
+template <typename Real>
+__global__ void chains_rows_kernel(const ChainDesc* __restrict__ cd) {
+    const ChainDesc& d = cd[blockIdx.x];
+    gather_rows<Real>((const Real*)d.x, d.kdim, d.p, d.traceRows, d.rows);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kGramThreads) void chains_gram_kernel(const ChainDesc* __restrict__ cd) {
+    __shared__ double red[kGramThreads];
+    const ChainDesc& d = cd[blockIdx.y];
+    if ((int)blockIdx.x >= d.p * d.p) return;  // the same for the whole block
+    gram_entry<Real>((const Real*)d.x, d.n, d.kdim, d.p, (int)blockIdx.x / d.p, (int)blockIdx.x % d.p, d.gram, red);
+}
+
+struct BatchHmc {
+    struct Chain {
+        int n = 0, p = 0;
+        double tau = 0.0, eps = 0.0, U = 0.0, logLik = 0.0, quad = 0.0;
+        std::vector<double> trait;
+        mdsgen::Stream rng;
+        double daMu = 0.0, daH = 0.0, daLogEpsBar = 0.0;
+        long long accepted = 0, proposedTau = 0, acceptedTau = 0, evaluations = 1;
+        size_t xBytes = 0, momOffset = 0, xSaveOffset = 0, locOffset = 0;
+        bool proposing = false;
+        double savedTau = 0.0;
+        PrecisionWidth width;
+        explicit Chain(uint64_t seed) : rng(seed, 77) {}
+    };
+
+    Batch& b;
+    mds_hip_hmc_options opt;
+    int C = 0, maxRows = 0, maxP = 0;
+    std::vector<Chain> ch;
+    double* dMom = nullptr;
+    double* dGradSave = nullptr;
+    char* dXSave = nullptr;
+    double* dParams = nullptr;   // [C][kHmcParamCount]
+    double* dEnergy = nullptr;   // [C][2]
+    double* dRows = nullptr;     // [C][kTraceRows * kMaxDim]
+    double* dGram = nullptr;     // [C][kMaxDim * kMaxDim]
+    ChainDesc* dChains = nullptr;
+    double* hParams = nullptr;   // pinned copies
+    double* hEnergy0 = nullptr;
+    double* hEnergy = nullptr;
+    double* hSums = nullptr;
+    double* hRows = nullptr;
+    double* hGram = nullptr;
+    hipEvent_t uploadDone = nullptr;
+    bool uploadPending = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    BatchConfig savedCfg;
+
+    BatchHmc(Batch& batch, const mds_hip_hmc_options& options) : b(batch), opt(options), savedCfg(batch.cfg) {
+        if (opt.iterations < 1 || opt.leapfrogSteps < 1 || opt.burnIn < 0 || opt.burnIn >= opt.iterations)
+            throw Error(MDS_HIP_ERR_ARGUMENT, "invalid iteration counts");
+        if (!(opt.trajectory > 0.0)) throw Error(MDS_HIP_ERR_ARGUMENT, "trajectory must be > 0");
+        if (opt.traitInvWeight < 1) throw Error(MDS_HIP_ERR_ARGUMENT, "traitInvWeight must be >= 1");
+        if (opt.useSurrogate || opt.collectSnapshots || opt.surrogate)
+            throw Error(MDS_HIP_ERR_ARGUMENT, "the batched sampler runs exact gradients only (no surrogate)");
+        C = (int)b.e.size();
+        size_t momCount = 0, xSaveBytes = 0, locCount = 0;
+        for (int c = 0; c < C; ++c) {
+            const BatchEntry& en = b.e[c];
+            if (!en.dataSet) throw Error(MDS_HIP_ERR_STATE, "pairwise data not set for dataset " + std::to_string(c));
+            if (!en.precisionSet) throw Error(MDS_HIP_ERR_STATE, "precision not set for dataset " + std::to_string(c));
+            Chain k(opt.seed + (uint64_t)c);
+            k.n = en.N;
+            k.p = en.P;
+            k.tau = en.tau;
+            k.trait.assign((size_t)k.p * k.p, 0.0);
+            for (int d = 0; d < k.p; ++d) k.trait[d * k.p + d] = 1.0;  // solve(traitVcv) with traitVcv = I_P
+            k.eps = opt.stepSize > 0.0 ? opt.stepSize : opt.trajectory / opt.leapfrogSteps;
+            k.daMu = std::log(10.0 * k.eps);
+            k.width.w = opt.precisionProposalWidth;
+            k.width.adapt = opt.adaptPrecisionWidth != 0;
+            k.xBytes = (size_t)en.nb * kTile * en.kdim * b.realSize;
+            k.momOffset = momCount;
+            k.xSaveOffset = xSaveBytes;
+            k.locOffset = locCount;
+            momCount += (size_t)k.n * k.p;
+            xSaveBytes += (k.xBytes + 255) / 256 * 256;
+            locCount += (size_t)k.n * k.p;
+            maxRows = std::max(maxRows, k.n);
+            maxP = std::max(maxP, k.p);
+            ch.push_back(std::move(k));
+        }
+        try {
+            check(hipMalloc((void**)&dMom, momCount * sizeof(double)), "hipMalloc(momentum)");
+            check(hipMalloc((void**)&dGradSave, momCount * sizeof(double)), "hipMalloc(gradient copies)");
+            check(hipMalloc((void**)&dXSave, xSaveBytes), "hipMalloc(location copies)");
+            check(hipMalloc((void**)&dParams, (size_t)C * kHmcParamCount * sizeof(double)), "hipMalloc(parameters)");
+            check(hipMalloc((void**)&dEnergy, (size_t)C * 2 * sizeof(double)), "hipMalloc(energies)");
+            check(hipMalloc((void**)&dRows, (size_t)C * kTraceRows * kMaxDim * sizeof(double)), "hipMalloc(rows)");
+            check(hipMalloc((void**)&dGram, (size_t)C * kMaxDim * kMaxDim * sizeof(double)), "hipMalloc(gram)");
+            check(hipMalloc((void**)&dChains, (size_t)C * sizeof(ChainDesc)), "hipMalloc(chains)");
+            check(hipHostMalloc((void**)&hParams, (size_t)C * kHmcParamCount * sizeof(double), hipHostMallocDefault),
+                  "hipHostMalloc");
+            check(hipHostMalloc((void**)&hEnergy0, (size_t)C * 2 * sizeof(double), hipHostMallocDefault),
+                  "hipHostMalloc");
+            check(hipHostMalloc((void**)&hEnergy, (size_t)C * 2 * sizeof(double), hipHostMallocDefault),
+                  "hipHostMalloc");
+            check(hipHostMalloc((void**)&hSums, (size_t)C * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+            check(hipHostMalloc((void**)&hRows, (size_t)C * kTraceRows * kMaxDim * sizeof(double),
+                                hipHostMallocDefault), "hipHostMalloc");
+            check(hipHostMalloc((void**)&hGram, (size_t)C * kMaxDim * kMaxDim * sizeof(double), hipHostMallocDefault),
+                  "hipHostMalloc");
+            check(hipEventCreateWithFlags(&uploadDone, hipEventDisableTiming), "hipEventCreate");
+            std::memset(hParams, 0, (size_t)C * kHmcParamCount * sizeof(double));
+            std::vector<ChainDesc> desc(C);
+            for (int c = 0; c < C; ++c) {
+                ChainDesc& d = desc[c];
+                d.x = b.e[c].dX;
+                d.mom = dMom + ch[c].momOffset;
+                d.grad = b.e[c].dGrad;
+                d.params = dParams + (size_t)c * kHmcParamCount;
+                d.energy = dEnergy + 2 * c;
+                d.rows = dRows + (size_t)c * kTraceRows * kMaxDim;
+                d.gram = dGram + (size_t)c * kMaxDim * kMaxDim;
+                d.traceRows = trace_rows(ch[c].n);
+                d.seed = opt.seed + (uint64_t)c;
+                d.n = ch[c].n;
+                d.p = ch[c].p;
+                d.kdim = b.e[c].kdim;
+            }
+            check(hipMemcpy(dChains, desc.data(), (size_t)C * sizeof(ChainDesc), hipMemcpyHostToDevice),
+                  "hipMemcpy(chains)");
+            // one tile launch over every chain (no side streams: the trajectory
+            // is captured on the batch's stream), the precision read from each
+            // chain's parameter block
+            b.cfg.strategy = 0;
+            for (int c = 0; c < C; ++c) b.e[c].devTau = dParams + (size_t)c * kHmcParamCount + kHmcTau;
+            b.planDirty = true;
+            b.buildPlan();
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+
+    ~BatchHmc() { release(); }
+
+    void release() {
+        if (b.stream) (void)hipStreamSynchronize(b.stream);
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graphExec = nullptr;
+        graph = nullptr;
+        (void)hipFree(dMom);
+        (void)hipFree(dGradSave);
+        (void)hipFree(dXSave);
+        (void)hipFree(dParams);
+        (void)hipFree(dEnergy);
+        (void)hipFree(dRows);
+        (void)hipFree(dGram);
+        (void)hipFree(dChains);
+        (void)hipHostFree(hParams);
+        (void)hipHostFree(hEnergy0);
+        (void)hipHostFree(hEnergy);
+        (void)hipHostFree(hSums);
+        (void)hipHostFree(hRows);
+        (void)hipHostFree(hGram);
+        if (uploadDone) (void)hipEventDestroy(uploadDone);
+        dMom = dGradSave = dParams = dEnergy = dRows = dGram = nullptr;
+        dXSave = nullptr;
+        dChains = nullptr;
+        hParams = hEnergy0 = hEnergy = hSums = hRows = hGram = nullptr;
+        uploadDone = nullptr;
+        // the batch evaluates with the host precisions again
+        for (BatchEntry& en : b.e) en.devTau = nullptr;
+        b.cfg = savedCfg;
+        b.planDirty = true;
+    }
+
+    // The pinned block is rewritten only after the previous upload from it has run.
+    void uploadParams() {
+        if (uploadPending) check(hipEventSynchronize(uploadDone), "hipEventSynchronize(parameters)");
+        for (int c = 0; c < C; ++c) {
+            double* h = hParams + (size_t)c * kHmcParamCount;
+            h[kHmcTau] = ch[c].tau;
+            h[kHmcEps] = ch[c].eps;
+            for (int k = 0; k < ch[c].p * ch[c].p; ++k) h[kHmcTrait + k] = ch[c].trait[k];
+        }
+        check(hipMemcpyAsync(dParams, hParams, (size_t)C * kHmcParamCount * sizeof(double), hipMemcpyHostToDevice,
+                             b.stream),
+              "hipMemcpy(parameters)");
+        check(hipEventRecord(uploadDone, b.stream), "hipEventRecord(parameters)");
+        uploadPending = true;
+    }
+
+    template <int FORM>
+    void kick() {
+        const dim3 grid((maxRows + kChainThreads - 1) / kChainThreads, C);
+        if (b.fp32)
+            hipLaunchKernelGGL((chains_kick_kernel<float, FORM>), grid, dim3(kChainThreads), 0, b.stream, dChains);
+        else
+            hipLaunchKernelGGL((chains_kick_kernel<double, FORM>), grid, dim3(kChainThreads), 0, b.stream, dChains);
+        check(hipGetLastError(), "chains kick kernel");
+    }
+
+    void energies() {
+        if (b.fp32)
+            hipLaunchKernelGGL(chains_energy_kernel<float>, dim3(C), dim3(kEnergyThreads), 0, b.stream, dChains);
+        else
+            hipLaunchKernelGGL(chains_energy_kernel<double>, dim3(C), dim3(kEnergyThreads), 0, b.stream, dChains);
+        check(hipGetLastError(), "chains energy kernel");
+    }
+
+    // Every chain's likelihood and gradient at its current locations.
+    void evaluateAll() { b.enqueue(); }
+
+    void trajectory() {
+        kick<kKickDrift>();
+        for (int s = 0; s < opt.leapfrogSteps; ++s) {
+            evaluateAll();
+            if (s + 1 < opt.leapfrogSteps)
+                kick<kKick2Drift>();
+            else
+                kick<kKick>();
+        }
+    }
+
+    // The whole trajectory of every chain as one graph; every exit ends the capture.
+    void buildGraph() {
+        check(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipGraph_t g = nullptr;
+        try {
+            trajectory();
+        } catch (...) {
+            if (hipStreamEndCapture(b.stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            throw;
+        }
+        check(hipStreamEndCapture(b.stream, &g), "hipStreamEndCapture");
+        graph = g;
+        check(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+
+    double logPrior(const Chain& k, double quad) const { return matrix_normal_log_prior(quad, k.n, k.p, k.trait); }
+    double logLik(const Chain& k, double rawSum, double t) const {
+        return mds_log_likelihood(rawSum, t, k.n, b.truncated);
+    }
+
+    void readSums() {
+        check(hipMemcpyAsync(hSums, b.dSums, (size_t)C * sizeof(double), hipMemcpyDeviceToHost, b.stream),
+              "hipMemcpy(sums)");
+    }
+
+    void saveGradient(int c) {
+        const size_t bytes = (size_t)ch[c].n * ch[c].p * sizeof(double);
+        check(hipMemcpyAsync(dGradSave + ch[c].momOffset, b.e[c].dGrad, bytes, hipMemcpyDeviceToDevice, b.stream),
+              "hipMemcpy(gradient copy)");
+    }
+
+    void restoreGradient(int c) {
+        const size_t bytes = (size_t)ch[c].n * ch[c].p * sizeof(double);
+        check(hipMemcpyAsync(b.e[c].dGrad, dGradSave + ch[c].momOffset, bytes, hipMemcpyDeviceToDevice, b.stream),
+              "hipMemcpy(gradient restore)");
+    }
+
+    void run(const double* x0, double* traces, double* finalX, mds_hip_hmc_stats* stats) {
+        using clock = std::chrono::steady_clock;
+        const auto tStart = clock::now();
+        const int kept = opt.iterations - opt.burnIn;
+        for (int c = 0; c < C; ++c) b.updateLocations(c, x0 + ch[c].locOffset, (size_t)ch[c].n * ch[c].p);
+        uploadParams();
+        evaluateAll();
+        size_t momTotal = 0;
+        for (const Chain& k : ch) momTotal += (size_t)k.n * k.p;
+        check(hipMemsetAsync(dMom, 0, momTotal * sizeof(double), b.stream), "hipMemset(momentum)");
+        energies();
+        check(hipMemcpyAsync(hEnergy, dEnergy, (size_t)C * 2 * sizeof(double), hipMemcpyDeviceToHost, b.stream),
+              "hipMemcpy(energies)");
+        readSums();
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        for (int c = 0; c < C; ++c) {
+            Chain& k = ch[c];
+            k.logLik = logLik(k, hSums[c], k.tau);
+            k.quad = hEnergy[2 * c + 1];
+            k.U = -(k.logLik + logPrior(k, k.quad));
+        }
+        if (opt.useGraph) buildGraph();
+        const double daGamma = 0.05, daT0 = 10.0, daKappa = 0.75;
+        auto tSampling = clock::now();
+        int maxPairs = 0;
+        for (const Chain& k : ch) maxPairs = std::max(maxPairs, (k.n * k.p + 1) / 2);
+        const dim3 momGrid((maxPairs + kChainThreads - 1) / kChainThreads, C);
+        for (int it = 0; it < opt.iterations; ++it) {
+            if (it == opt.burnIn) {
+                tSampling = clock::now();
+                if (opt.adaptStepSize && opt.burnIn > 0) {
+                    for (Chain& k : ch) k.eps = std::exp(k.daLogEpsBar);
+                    uploadParams();
+                }
+            }
+            hipLaunchKernelGGL(chains_momentum_kernel, momGrid, dim3(kChainThreads), 0, b.stream, dChains,
+                               (uint64_t)it);
+            check(hipGetLastError(), "chains momentum kernel");
+            energies();
+            check(hipMemcpyAsync(hEnergy0, dEnergy, (size_t)C * 2 * sizeof(double), hipMemcpyDeviceToHost, b.stream),
+                  "hipMemcpy(energies)");
+            for (int c = 0; c < C; ++c) {
+                check(hipMemcpyAsync(dXSave + ch[c].xSaveOffset, b.e[c].dX, ch[c].xBytes, hipMemcpyDeviceToDevice,
+                                     b.stream),
+                      "hipMemcpy(location copy)");
+                saveGradient(c);
+            }
+            if (graphExec)
+                check(hipGraphLaunch(graphExec, b.stream), "hipGraphLaunch");
+            else
+                trajectory();
+            energies();
+            check(hipMemcpyAsync(hEnergy, dEnergy, (size_t)C * 2 * sizeof(double), hipMemcpyDeviceToHost, b.stream),
+                  "hipMemcpy(energies)");
+            readSums();
+            check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+            bool adapted = false;
+            for (int c = 0; c < C; ++c) {
+                Chain& k = ch[c];
+                const double K0 = hEnergy0[2 * c], K1 = hEnergy[2 * c];
+                const double logLik1 = logLik(k, hSums[c], k.tau);
+                const double quad1 = hEnergy[2 * c + 1];
+                const double U1 = -(logLik1 + logPrior(k, quad1));
+                const double ratio = (k.U + K0) - (U1 + K1);  // R/Untitled.R:630-637
+                const double u = k.rng.uniform();
+                const bool acc = std::isfinite(ratio) && ratio > std::min(0.0, std::log(u));
+                if (acc) {
+                    k.U = U1;
+                    k.logLik = logLik1;
+                    k.quad = quad1;
+                    ++k.accepted;
+                } else {
+                    check(hipMemcpyAsync(b.e[c].dX, dXSave + k.xSaveOffset, k.xBytes, hipMemcpyDeviceToDevice,
+                                         b.stream),
+                          "hipMemcpy(location restore)");
+                    restoreGradient(c);
+                }
+                k.evaluations += opt.leapfrogSteps;
+                if (opt.adaptStepSize && it < opt.burnIn) {
+                    // dual averaging (Hoffman & Gelman 2014, sec. 3.2), per chain
+                    const double alpha = std::isfinite(ratio) ? std::min(1.0, std::exp(std::min(ratio, 0.0))) : 0.0;
+                    const double m = it + 1.0;
+                    k.daH = (1.0 - 1.0 / (m + daT0)) * k.daH + (opt.targetAccept - alpha) / (m + daT0);
+                    const double logEps = k.daMu - std::sqrt(m) / daGamma * k.daH;
+                    const double eta = std::pow(m, -daKappa);
+                    k.daLogEpsBar = eta * logEps + (1.0 - eta) * k.daLogEpsBar;
+                    k.eps = std::exp(logEps);
+                    adapted = true;
+                }
+            }
+            if (adapted) uploadParams();
+
+            // trace rows and X^T X of the kept locations (the precision step
+            // does not move them), read after the next wait
+            const bool traitStep = opt.learnTraitPrecision && it < opt.iterations - 1 &&
+                                   ((it + 1) % opt.traitInvWeight == 0);
+            const bool recordIt = it >= opt.burnIn && traces;
+            if (traitStep) {
+                if (b.fp32)
+                    hipLaunchKernelGGL(chains_gram_kernel<float>, dim3(maxP * maxP, C), dim3(kGramThreads), 0,
+                                       b.stream, dChains);
+                else
+                    hipLaunchKernelGGL(chains_gram_kernel<double>, dim3(maxP * maxP, C), dim3(kGramThreads), 0,
+                                       b.stream, dChains);
+                check(hipGetLastError(), "chains gram kernel");
+                check(hipMemcpyAsync(hGram, dGram, (size_t)C * kMaxDim * kMaxDim * sizeof(double),
+                                     hipMemcpyDeviceToHost, b.stream),
+                      "hipMemcpy(gram)");
+            }
+            if (recordIt) {
+                if (b.fp32)
+                    hipLaunchKernelGGL(chains_rows_kernel<float>, dim3(C), dim3(kTraceRows * kMaxDim), 0, b.stream,
+                                       dChains);
+                else
+                    hipLaunchKernelGGL(chains_rows_kernel<double>, dim3(C), dim3(kTraceRows * kMaxDim), 0, b.stream,
+                                       dChains);
+                check(hipGetLastError(), "chains rows kernel");
+                check(hipMemcpyAsync(hRows, dRows, (size_t)C * kTraceRows * kMaxDim * sizeof(double),
+                                     hipMemcpyDeviceToHost, b.stream),
+                      "hipMemcpy(rows)");
+            }
+
+            if (opt.learnPrecision) {
+                if (b.truncated) {
+                    // random-walk Metropolis on each chain's precision at its current locations
+                    bool any = false;
+                    for (int c = 0; c < C; ++c) {
+                        Chain& k = ch[c];
+                        const double lo = k.tau - k.width.w, hi = k.tau + k.width.w;
+                        const double tauStar = std::fabs(lo + (hi - lo) * k.rng.uniform());
+                        ++k.proposedTau;
+                        k.proposing = tauStar > 0.0;
+                        if (k.proposing) {
+                            k.savedTau = k.tau;
+                            k.tau = tauStar;
+                            saveGradient(c);
+                            any = true;
+                        }
+                    }
+                    if (any) {
+                        uploadParams();
+                        evaluateAll();
+                        readSums();
+                        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+                        bool restored = false;
+                        for (int c = 0; c < C; ++c) {
+                            Chain& k = ch[c];
+                            if (!k.proposing) continue;
+                            ++k.evaluations;
+                            const double logLikStar = logLik(k, hSums[c], k.tau);
+                            const double r = logLikStar - k.logLik;  // the prior does not involve tau
+                            const bool accTau = std::isfinite(r) && r > std::min(0.0, std::log(k.rng.uniform()));
+                            k.width.update(accTau, it < opt.burnIn);
+                            if (accTau) {
+                                // the pass at the new precision left its gradient in place
+                                k.logLik = logLikStar;
+                                k.U = -(k.logLik + logPrior(k, k.quad));
+                                ++k.acceptedTau;
+                                ++k.evaluations;
+                            } else {
+                                k.tau = k.savedTau;
+                                restoreGradient(c);
+                                restored = true;
+                            }
+                        }
+                        if (restored) uploadParams();
+                    }
+                } else {
+                    // Gibbs: tau | X ~ Gamma(1 + N(N-1)/4, C / 2) with C the unscaled
+                    // sum of squared residuals (the raw sum at the current X)
+                    evaluateAll();
+                    readSums();
+                    check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+                    std::vector<double> raw(hSums, hSums + C);
+                    for (int c = 0; c < C; ++c) {
+                        Chain& k = ch[c];
+                        const double pairs = (double)k.n * (k.n - 1) / 2.0;
+                        k.tau = gamma_draw(k.rng, 1.0 + 0.5 * pairs) / (0.5 * raw[c]);
+                        ++k.proposedTau;
+                        ++k.acceptedTau;
+                        k.evaluations += 2;
+                    }
+                    uploadParams();
+                    evaluateAll();
+                    for (int c = 0; c < C; ++c) {
+                        Chain& k = ch[c];
+                        k.logLik = logLik(k, raw[c], k.tau);
+                        k.U = -(k.logLik + logPrior(k, k.quad));
+                    }
+                }
+            }
+
+            if (traitStep || recordIt) check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+            if (traitStep) {
+                for (int c = 0; c < C; ++c) {
+                    Chain& k = ch[c];
+                    const double* gram = hGram + (size_t)c * kMaxDim * kMaxDim;
+                    std::vector<double> m((size_t)k.p * k.p, 0.0);  // T0 + X^T X, T0 = I_P
+                    for (int d = 0; d < k.p; ++d) m[d * k.p + d] = 1.0;
+                    for (int q = 0; q < k.p * k.p; ++q) m[q] += gram[q];
+                    const double df = (opt.priorDf > 0 ? opt.priorDf : k.p) + k.n;
+                    k.trait = wishart_draw(k.rng, df, spd_inverse(m, k.p), k.p);
+                    k.quad = 0.0;  // sum_i x_i^T T x_i = sum_ab T_ab (X^T X)_ab
+                    for (int q = 0; q < k.p * k.p; ++q) k.quad += k.trait[q] * gram[q];
+                    k.U = -(k.logLik + logPrior(k, k.quad));
+                }
+                uploadParams();
+            }
+            if (recordIt)
+                for (int c = 0; c < C; ++c) {
+                    const Chain& k = ch[c];
+                    double* row = traces + ((size_t)c * kept + (it - opt.burnIn)) * kHmcTraceColumns;
+                    write_trace_row(row, hRows + (size_t)c * kTraceRows * kMaxDim, k.p, -k.U, k.tau, k.trait);
+                }
+        }
+        check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        const auto tEnd = clock::now();
+        if (finalX) {
+            for (int c = 0; c < C; ++c) {
+                const Chain& k = ch[c];
+                const int kd = b.e[c].kdim;
+                std::vector<char> host((size_t)k.n * kd * b.realSize);
+                check(hipMemcpy(host.data(), b.e[c].dX, host.size(), hipMemcpyDeviceToHost), "hipMemcpy(locations)");
+                for (int i = 0; i < k.n; ++i)
+                    for (int d = 0; d < k.p; ++d)
+                        finalX[k.locOffset + (size_t)i * k.p + d] =
+                            b.fp32 ? (double)((const float*)host.data())[(size_t)i * kd + d]
+                                   : ((const double*)host.data())[(size_t)i * kd + d];
+            }
+        }
+        for (int c = 0; c < C; ++c) b.e[c].tau = ch[c].tau;  // the batch keeps each chain's final precision
+        if (stats)
+            for (int c = 0; c < C; ++c) {
+                const Chain& k = ch[c];
+                mds_hip_hmc_stats& st = stats[c];
+                std::memset(&st, 0, sizeof(st));
+                st.acceptRate = (double)k.accepted / opt.iterations;
+                st.precisionAcceptRate = k.proposedTau ? (double)k.acceptedTau / k.proposedTau : 0.0;
+                st.samplingSeconds = std::chrono::duration<double>(tEnd - tSampling).count();
+                st.totalSeconds = std::chrono::duration<double>(tEnd - tStart).count();
+                st.evaluations = (double)k.evaluations;
+                st.finalPrecision = k.tau;
+                st.stepSize = k.eps;
+                st.finalPotential = k.U;
+                st.exactIterations = opt.iterations;
+                st.precisionProposalWidth = k.width.w;
+            }
     }
 };
 
@@ -789,6 +1385,32 @@ int mds_hip_batch_get_config(mds_hip_batch_handle h, int* wavesPerBlock, int* bl
         if (blocksPerCU) *blocksPerCU = b.cfg.blocksPerCU;
         if (tilesPerItem) *tilesPerItem = b.cfg.tilesPerItem;
         if (strategy) *strategy = b.cfg.strategy;
+    });
+}
+
+int mds_hip_batch_share_pairwise_data(mds_hip_batch_handle h, int index, int sourceIndex) {
+    return bguard(h, [&](mdship::Batch& b) { b.sharePairwiseData(index, sourceIndex); });
+}
+
+int mds_hip_batch_hmc_run(mds_hip_batch_handle h, const mds_hip_hmc_options* options, const double* initialLocations,
+                          size_t length, double* traces, size_t traceLength, double* finalLocations,
+                          size_t finalLength, mds_hip_hmc_stats* stats, int statsCount) {
+    return bguard(h, [&](mdship::Batch& b) {
+        size_t locations = 0;
+        for (const mdship::BatchEntry& en : b.e) locations += (size_t)en.N * en.P;
+        if (!options || !initialLocations || length != locations)
+            throw mdship::Error(MDS_HIP_ERR_ARGUMENT,
+                                "batch hmc: options and the concatenated N_k x P_k initial locations required");
+        if (traces && traceLength != b.e.size() * (size_t)(options->iterations - options->burnIn) *
+                                         MDS_HIP_HMC_TRACE_COLUMNS)
+            throw mdship::Error(MDS_HIP_ERR_ARGUMENT, "batch hmc: trace length must be count * (iterations - burnIn) "
+                                                      "* columns");
+        if (finalLocations && finalLength != locations)
+            throw mdship::Error(MDS_HIP_ERR_ARGUMENT, "batch hmc: final locations length must be sum N_k P_k");
+        if (stats && statsCount != (int)b.e.size())
+            throw mdship::Error(MDS_HIP_ERR_ARGUMENT, "batch hmc: one stats entry per dataset");
+        mdship::BatchHmc hmc(b, *options);
+        hmc.run(initialLocations, traces, finalLocations, stats);
     });
 }
 

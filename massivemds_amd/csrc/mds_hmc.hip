@@ -37,72 +37,24 @@
 #include "mds_device.hpp"
 #include "mds_engine.hpp"
 #include "mds_hmc.hpp"
+#include "mds_hmc_device.hpp"
 #include "mds_mlp.hpp"
 
 namespace mdship {
 
 // ------------------------------------------------------------------ kernels
 
-// splitmix64 finaliser, as mdsgen::mix on the host.
-__device__ __forceinline__ uint64_t dmix(uint64_t z) {
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
-
 // Standard normal momenta from a counter-based stream keyed by (seed,
-// iteration): the same splitmix64 / Box-Muller construction as the host
-// generator, one pair per thread.
+// iteration), one pair per thread (momentum_pair).
 __global__ void momentum_kernel(double* __restrict__ mom, int count, uint64_t seed, uint64_t iteration) {
-    const int pair = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = 2 * pair;
-    if (k >= count) return;
-    const uint64_t state = dmix(seed * 0x100000001B3ULL + (iteration + 1) * 0xC2B2AE3D27D4EB4FULL);
-    const uint64_t z1 = dmix(state + (uint64_t)(k + 1) * 0x9E3779B97F4A7C15ULL);
-    const uint64_t z2 = dmix(state + (uint64_t)(k + 2) * 0x9E3779B97F4A7C15ULL);
-    const double u1 = ((double)(z1 >> 11) + 0.5) * 0x1.0p-53;
-    const double u2 = ((double)(z2 >> 11) + 0.5) * 0x1.0p-53;
-    const double r = sqrt(-2.0 * log(u1));
-    double s, c;
-    sincospi(2.0 * u2, &s, &c);
-    mom[k] = r * c;
-    if (k + 1 < count) mom[k + 1] = r * s;
+    momentum_pair(mom, count, seed, iteration, blockIdx.x * blockDim.x + threadIdx.x);
 }
-
-// Row i: g = grad log-likelihood + grad log prior = grad_ll - x_i T.
-// The reference's leapfrog (R/Untitled.R:611-620) takes two half kicks per
-// step, mom += eps / 2 * g before and after the drift x += eps * mom; the
-// kernel forms: KICK mom += halfEps * g; KICK_DRIFT the kick, then the drift;
-// KICK2_DRIFT the half kick ending one step and the one starting the next
-// (same locations, so the same g; two separate additions, as two launches
-// would make them), then the drift.
-enum KickForm { kKick = 0, kKickDrift = 1, kKick2Drift = 2 };
 
 template <typename Real, int FORM>
 __global__ void kick_kernel(Real* __restrict__ x, double* __restrict__ mom, const double* __restrict__ grad,
                             const double* __restrict__ params, int n, int p, int kdim) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double eps = params[kHmcEps];
-    const double halfEps = 0.5 * eps;
-    const double* T = params + kHmcTrait;
-    Real* xi = x + (int64_t)i * kdim;
-    for (int d = 0; d < p; ++d) {
-        double prior = 0.0;
-        for (int e = 0; e < p; ++e) prior += (double)xi[e] * T[e * p + d];
-        const double g = grad[(int64_t)i * p + d] - prior;
-        double m = mom[(int64_t)i * p + d];
-        m = fma(halfEps, g, m);
-        if (FORM == kKick2Drift) m = fma(halfEps, g, m);
-        mom[(int64_t)i * p + d] = m;
-    }
-    if (FORM != kKick) {
-        for (int d = 0; d < p; ++d) xi[d] = (Real)((double)xi[d] + eps * mom[(int64_t)i * p + d]);
-    }
+    kick_row<Real, FORM>(x, mom, grad, params, n, p, kdim, blockIdx.x * blockDim.x + threadIdx.x);
 }
-
-// out[0] = sum mom^2 / 2, out[1] = sum_i x_i^T T x_i (fixed-order reductions).
-constexpr int kEnergyThreads = 512;
 
 template <typename Real>
 __global__ __launch_bounds__(kEnergyThreads) void energy_kernel(const Real* __restrict__ x,
@@ -110,62 +62,20 @@ __global__ __launch_bounds__(kEnergyThreads) void energy_kernel(const Real* __re
                                                                 const double* __restrict__ params, int n, int p,
                                                                 int kdim, double* __restrict__ out) {
     __shared__ double red[2][kEnergyThreads];
-    const double* T = params + kHmcTrait;
-    double k = 0.0, q = 0.0;
-    for (int i = threadIdx.x; i < n; i += kEnergyThreads) {
-        const Real* xi = x + (int64_t)i * kdim;
-        for (int d = 0; d < p; ++d) {
-            const double m = mom[(int64_t)i * p + d];
-            k += 0.5 * m * m;
-            double row = 0.0;
-            for (int e = 0; e < p; ++e) row += (double)xi[e] * T[e * p + d];
-            q += row * (double)xi[d];
-        }
-    }
-    red[0][threadIdx.x] = k;
-    red[1][threadIdx.x] = q;
-    __syncthreads();
-    for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + s];
-            red[1][threadIdx.x] += red[1][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        out[0] = red[0][0];
-        out[1] = red[1][0];
-    }
+    energy_block<Real>(x, mom, params, n, p, kdim, out, red);
 }
 
-// The trace's rows (kTraceRows of them, indices in rows), widened to double:
-// out[k * p + d].
 template <typename Real>
 __global__ void gather_rows_kernel(const Real* __restrict__ x, int kdim, int p, TraceRows rows,
                                    double* __restrict__ out) {
-    const int t = threadIdx.x;
-    if (t < kTraceRows * p) out[t] = (double)x[(int64_t)rows.r[t / p] * kdim + t % p];
+    gather_rows(x, kdim, p, rows, out);
 }
-
-// X^T X: workgroup (a, b) sums x_ia x_ib over i in a fixed order (strided
-// thread sums, then a fixed tree), so entries (a, b) and (b, a) agree bitwise.
-constexpr int kGramThreads = 256;
 
 template <typename Real>
 __global__ __launch_bounds__(kGramThreads) void gram_kernel(const Real* __restrict__ x, int n, int kdim, int p,
                                                             double* __restrict__ out) {
     __shared__ double red[kGramThreads];
-    const int a = blockIdx.x / p, b = blockIdx.x % p;
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kGramThreads)
-        acc += (double)x[(int64_t)i * kdim + a] * (double)x[(int64_t)i * kdim + b];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = kGramThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[a * p + b] = red[0];
+    gram_entry<Real>(x, n, kdim, p, blockIdx.x / p, blockIdx.x % p, out, red);
 }
 
 // ------------------------------------------------------------------ host
@@ -191,7 +101,7 @@ double gamma_draw(mdsgen::Stream& s, double shape) {
 }
 
 // Cholesky factor (lower) of a symmetric positive-definite p x p matrix.
-static bool cholesky(const std::vector<double>& a, int p, std::vector<double>& l) {
+bool cholesky(const std::vector<double>& a, int p, std::vector<double>& l) {
     l.assign((size_t)p * p, 0.0);
     for (int i = 0; i < p; ++i)
         for (int j = 0; j <= i; ++j) {
@@ -207,7 +117,7 @@ static bool cholesky(const std::vector<double>& a, int p, std::vector<double>& l
     return true;
 }
 
-static std::vector<double> spd_inverse(const std::vector<double>& a, int p) {
+std::vector<double> spd_inverse(const std::vector<double>& a, int p) {
     std::vector<double> l;
     if (!cholesky(a, p, l)) throw Error(MDS_HIP_ERR_STATE, "matrix not positive definite");
     // inverse of L, then (L^-1)^T L^-1
@@ -230,7 +140,7 @@ static std::vector<double> spd_inverse(const std::vector<double>& a, int p) {
     return inv;
 }
 
-static double log_det_spd(const std::vector<double>& a, int p) {
+double log_det_spd(const std::vector<double>& a, int p) {
     std::vector<double> l;
     if (!cholesky(a, p, l)) throw Error(MDS_HIP_ERR_STATE, "matrix not positive definite");
     double s = 0.0;
@@ -293,10 +203,7 @@ Hmc::Hmc(Engine& engine, const mds_hip_hmc_options& options) : e(engine), opt(op
     check(hipHostMalloc((void**)&hRows, kTraceRows * kMaxDim * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
     check(hipHostMalloc((void**)&hGram, kMaxDim * kMaxDim * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
     check(hipEventCreateWithFlags(&uploadDone, hipEventDisableTiming), "hipEventCreate");
-    {
-        const int r[kTraceRows] = {0, 1 % n, n / 4, n / 3, n / 2, 2 * n / 3, 3 * n / 4, n - 1};
-        for (int k = 0; k < kTraceRows; ++k) traceRows.r[k] = r[k];
-    }
+    traceRows = trace_rows(n);
     std::memset(hParams, 0, kHmcParamCount * sizeof(double));
     tau = e.precision;
     trait.assign((size_t)p * p, 0.0);
@@ -432,17 +339,21 @@ void Hmc::readEnergies() {
     check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
 }
 
-double Hmc::logLikelihood(double rawSum, double t) const {
+double mds_log_likelihood(double rawSum, double t, int n, bool truncated) {
     // rawSum = 0.5 * sum of tile partials; truncated: sumOfIncrements itself,
     // otherwise scaled by 0.5 tau (src/NewMultiDimensionalScaling.hpp:198-202)
-    const double s = e.truncated ? rawSum : 0.5 * t * rawSum;
+    const double s = truncated ? rawSum : 0.5 * t * rawSum;
     const double pairs = (double)n * (n - 1) / 2.0;
     return 0.5 * (std::log(t) - std::log(2.0 * M_PI)) * pairs - s;
 }
 
-double Hmc::logPrior(double quad) const {
+double matrix_normal_log_prior(double quad, int n, int p, const std::vector<double>& trait) {
     return -0.5 * quad - 0.5 * n * p * std::log(2.0 * M_PI) + 0.5 * n * log_det_spd(trait, p);
 }
+
+double Hmc::logLikelihood(double rawSum, double t) const { return mds_log_likelihood(rawSum, t, n, e.truncated); }
+
+double Hmc::logPrior(double quad) const { return matrix_normal_log_prior(quad, n, p, trait); }
 
 // Enqueues the trace rows (record) and X^T X (trait) of the current
 // locations into pinned host memory; the caller synchronises before reading.
@@ -483,6 +394,18 @@ void Hmc::copyLocationsToHost(std::vector<double>& out) {
 // rows: the trace rows in the order of traceRows (0, 1, n/4, n/3, n/2,
 // 2n/3, 3n/4, n - 1), p values each
 void Hmc::record(double* row, const double* rows, double logPost) const {
+    write_trace_row(row, rows, p, logPost, tau, trait);
+}
+
+TraceRows trace_rows(int n) {
+    TraceRows t;
+    const int r[kTraceRows] = {0, 1 % n, n / 4, n / 3, n / 2, 2 * n / 3, 3 * n / 4, n - 1};
+    for (int k = 0; k < kTraceRows; ++k) t.r[k] = r[k];
+    return t;
+}
+
+void write_trace_row(double* row, const double* rows, int p, double logPost, double tau,
+                     const std::vector<double>& trait) {
     row[0] = logPost;
     row[1] = tau;
     double tr = 0.0;
@@ -531,6 +454,9 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
     long long accepted = 0, proposedTau = 0, acceptedTau = 0, evaluations = 1;
     long long surrogateIts = 0, exactIts = 0, surrogateAccepted = 0;
     int windowIts = 0, windowAcc = 0, fallbackLeft = 0, failedWindows = 0;
+    PrecisionWidth width;
+    width.w = opt.precisionProposalWidth;
+    width.adapt = opt.adaptPrecisionWidth != 0;
     auto tSampling = clock::now();
     const int threads = 256;
     for (int it = 0; it < opt.iterations; ++it) {
@@ -648,7 +574,7 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             }
             if (e.truncated) {
                 // random-walk Metropolis at the current locations
-                const double lo = tau - opt.precisionProposalWidth, hi = tau + opt.precisionProposalWidth;
+                const double lo = tau - width.w, hi = tau + width.w;
                 const double tauStar = std::fabs(lo + (hi - lo) * rng.uniform());
                 ++proposedTau;
                 if (tauStar > 0.0) {
@@ -662,7 +588,9 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
                     check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
                     const double logLikStar = logLikelihood(hScalars[2], tauStar);
                     const double r = logLikStar - logLik;  // the prior does not involve tau
-                    if (std::isfinite(r) && r > std::min(0.0, std::log(rng.uniform()))) {
+                    const bool accTau = std::isfinite(r) && r > std::min(0.0, std::log(rng.uniform()));
+                    width.update(accTau, it < opt.burnIn);
+                    if (accTau) {
                         logLik = logLikStar;
                         U = -(logLik + logPrior(quad));
                         ++acceptedTau;
@@ -733,6 +661,7 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
         stats->exactIterations = (double)exactIts;
         stats->snapshots = sur ? (double)(sur->store.count() - startSnapshots) : 0.0;
         stats->surrogateAcceptRate = surrogateIts ? (double)surrogateAccepted / surrogateIts : 0.0;
+        stats->precisionProposalWidth = width.w;
     }
 }
 
@@ -761,6 +690,7 @@ int mds_hip_hmc_default_options(mds_hip_hmc_options* o) {
     o->collectSnapshots = 0;
     o->fallbackAcceptRate = 0.2;
     o->fallbackWindow = 20;
+    o->adaptPrecisionWidth = 0;
     return MDS_HIP_OK;
 }
 

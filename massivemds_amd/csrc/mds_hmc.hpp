@@ -1,8 +1,10 @@
 // Device-resident HMC sampler over an Engine (see mds_hmc.hip).
 #pragma once
 
+#include <cmath>
 #include <vector>
 
+#include "mds_datagen.hpp"
 #include "mds_engine.hpp"
 #include "mds_hip.h"
 #include "mds_mlp.hpp"
@@ -22,6 +24,44 @@ constexpr int kTraceRows = 8;
 struct TraceRows {
     int r[kTraceRows];
 };
+
+// Width w of the precision's random-walk proposal |U(tau - w, tau + w)|
+// (R/Untitled.R:668-691).  With adaptation, during burn-in only, log w moves
+// by (accepted - 0.44) / sqrt(proposals so far) after each evaluated
+// proposal (Robbins-Monro toward the 0.44 acceptance of a one-dimensional
+// random walk); after burn-in w stays fixed, so the sampled part of the
+// chain runs one fixed kernel.
+struct PrecisionWidth {
+    double w = 0.0;
+    bool adapt = false;
+    long long proposals = 0;
+    void update(bool accepted, bool burnIn) {
+        if (!adapt || !burnIn) return;
+        ++proposals;
+        w *= std::exp(((accepted ? 1.0 : 0.0) - 0.44) / std::sqrt((double)proposals));
+    }
+};
+
+// Host helpers of the precision and trait-precision updates (mds_hmc.hip).
+// Marsaglia & Tsang (2000) gamma variate of the given shape (scale 1).
+double gamma_draw(mdsgen::Stream& s, double shape);
+// Cholesky factor (lower) of a symmetric positive-definite p x p matrix;
+// false when it is not positive definite.
+bool cholesky(const std::vector<double>& a, int p, std::vector<double>& l);
+std::vector<double> spd_inverse(const std::vector<double>& a, int p);
+double log_det_spd(const std::vector<double>& a, int p);
+// Wishart(df, scale) by the Bartlett decomposition.
+std::vector<double> wishart_draw(mdsgen::Stream& s, double df, const std::vector<double>& scale, int p);
+// The sampler's log-likelihood from the engine's raw sum (R/mds.R:24-28) and
+// the matrix-normal log prior with row covariance I_N (R/Untitled.R:68-95),
+// quad = sum_i x_i^T T x_i.
+double mds_log_likelihood(double rawSum, double t, int n, bool truncated);
+double matrix_normal_log_prior(double quad, int n, int p, const std::vector<double>& trait);
+// The trace's location rows of an N-location run, and one trace row from
+// them (MDS_HIP_HMC_TRACE_COLUMNS values).
+TraceRows trace_rows(int n);
+void write_trace_row(double* row, const double* rows, int p, double logPost, double tau,
+                     const std::vector<double>& trait);
 
 struct Hmc {
     Engine& e;

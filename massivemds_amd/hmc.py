@@ -44,6 +44,7 @@ class HmcOptions(ctypes.Structure):
         ("collectSnapshots", ctypes.c_int),
         ("fallbackAcceptRate", ctypes.c_double),
         ("fallbackWindow", ctypes.c_int),
+        ("adaptPrecisionWidth", ctypes.c_int),
     ]
 
 
@@ -61,6 +62,7 @@ class HmcStats(ctypes.Structure):
         ("exactIterations", ctypes.c_double),
         ("snapshots", ctypes.c_double),
         ("surrogateAcceptRate", ctypes.c_double),
+        ("precisionProposalWidth", ctypes.c_double),
     ]
 
 
@@ -128,12 +130,14 @@ def hmcsampler(n_iter: int, burnIn: int = 0, data=None, latentDimension: int = 2
                learnTraitPrec: bool = False, trajectory: float = 0.2, traitInvWeight: int = 1,
                mdsPrecision: float = 1.0, truncation: bool = True, gpu: int = 1, single: bool = False,
                leapfrog: int = 20, seed: int = 666, useGraph: bool = True, precisionWidth: float = 0.01,
-               initial=None):
+               initial=None, adaptPrecisionWidth: bool = False):
     """R/Untitled.R:477-729 on the GPU.  Returns a dict with the trace of the
     invariant functionals after burn-in (``target`` = minus the log posterior,
     as R's Target), ``Time`` (seconds after burn-in), ``acprat`` (acceptance
     rate), ``precision``, ``locations`` (final), ``ess`` per functional and
-    ``min_ess_per_s``."""
+    ``min_ess_per_s``.  ``adaptPrecisionWidth`` adapts the width of the
+    precision's random-walk proposal during burn-in (the reference keeps
+    ``precisionWidth`` fixed)."""
     if data is None:
         raise ValueError("No data found.")
     if gpu <= 0:
@@ -156,6 +160,7 @@ def hmcsampler(n_iter: int, burnIn: int = 0, data=None, latentDimension: int = 2
     opt.learnTraitPrecision = int(bool(learnTraitPrec))
     opt.traitInvWeight = int(traitInvWeight)
     opt.precisionProposalWidth = float(precisionWidth)
+    opt.adaptPrecisionWidth = int(bool(adaptPrecisionWidth))
     opt.seed = int(seed)
     opt.useGraph = int(bool(useGraph))
     trace, final, stats = run_hmc(eng, x0, opt)
@@ -170,4 +175,5 @@ def hmcsampler(n_iter: int, burnIn: int = 0, data=None, latentDimension: int = 2
         "precision": trace[:, 1], "locations": final, "ess": ess, "min_ess": min_ess,
         "min_ess_per_s": min_ess / stats.samplingSeconds if stats.samplingSeconds > 0 else float("nan"),
         "evaluations": stats.evaluations, "total_seconds": stats.totalSeconds, "step_size": stats.stepSize,
+        "precision_width": stats.precisionProposalWidth,
     }
