@@ -54,7 +54,7 @@ struct Engine {
     int cur = 0;
     uint32_t* dTiles = nullptr;
     double* dTileSums = nullptr;
-    double* dPart = nullptr;
+    void* dPart = nullptr;     // gradient partials in the engine's real type
     double* dGrad = nullptr;
     double* dSum = nullptr;
     double* hGrad = nullptr;   // pinned
@@ -168,8 +168,7 @@ struct Engine {
                                  hipMemcpyHostToDevice, stream),
                   "hipMemcpy(tiles)");
             check(hipMalloc((void**)&dTileSums, tiles.size() * sizeof(double)), "hipMalloc(sums)");
-            check(hipMalloc((void**)&dPart, (size_t)nb * nb * kdim * kTile * sizeof(double)),
-                  "hipMalloc(partials)");
+            check(hipMalloc(&dPart, (size_t)nb * nb * kdim * kTile * realSize), "hipMalloc(partials)");
             check(hipMalloc((void**)&dGrad, (size_t)N * P * sizeof(double)), "hipMalloc(gradient)");
             check(hipMalloc((void**)&dSum, sizeof(double)), "hipMalloc(sum)");
             check(hipHostMalloc((void**)&hGrad, (size_t)N * P * sizeof(double), hipHostMallocDefault),
@@ -220,7 +219,8 @@ struct Engine {
         hRowPartial = nullptr;
         dDelta = dX[0] = dX[1] = nullptr;
         dTiles = nullptr;
-        dTileSums = dPart = dGrad = dSum = hGrad = hGradStored = hSum = nullptr;
+        dTileSums = dGrad = dSum = hGrad = hGradStored = hSum = nullptr;
+        dPart = nullptr;
         hX = nullptr;
         stream = nullptr;
     }
@@ -445,7 +445,7 @@ struct Engine {
     // kernels read the precision from this device address instead of the
     // engine's value (for captured graphs whose precision changes between
     // replays).
-    void launch_tile_pass(bool gradient, const double* devTau, double* tileSums, double* part) {
+    void launch_tile_pass(bool gradient, const double* devTau, double* tileSums, void* part) {
         TileLaunch a;
         a.delta = dDelta;
         a.x = dX[cur];
@@ -466,10 +466,11 @@ struct Engine {
 
     // The reduce kernel: the gradient rows into grad (when gradient) and
     // (0.5 * the sum of the tile sums) * sumScale into *sumOut.
-    void launch_reduce_pass(bool gradient, const double* tileSums, const double* part, double* grad, double* sumOut,
+    void launch_reduce_pass(bool gradient, const double* tileSums, const void* part, double* grad, double* sumOut,
                             double sumScale) {
         ReduceLaunch r;
         r.part = part;
+        r.fp32 = fp32;
         r.tileSums = tileSums;
         r.numTiles = numTiles;
         r.nb = nb;

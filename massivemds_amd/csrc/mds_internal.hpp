@@ -38,12 +38,13 @@ struct TileLaunch {
     bool symmetric;
     bool gradient;
     double* tileSums;        // [numTiles]
-    double* part;            // [nb][nb][kdim][64] gradient partials
+    void* part;              // [nb][nb][kdim][64] gradient partials, Real
     const double* devTau = nullptr;  // if set, the precision is read from here
 };
 
 struct ReduceLaunch {
-    const double* part;
+    const void* part;        // gradient partials in the engine's real type (fp32: float)
+    bool fp32;
     const double* tileSums;
     int numTiles;
     int nb;

@@ -31,7 +31,7 @@ template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 __global__ __launch_bounds__(kBlockThreads) void tile_kernel(
     const Real* __restrict__ delta, const Real* __restrict__ x,
     const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
-    double tau, double* __restrict__ tileSums, double* __restrict__ part, const double* __restrict__ devTau) {
+    double tau, double* __restrict__ tileSums, Real* __restrict__ part, const double* __restrict__ devTau) {
     __shared__ __align__(16) Real lnPhiTab[TRUNC ? TableOf<Real>::kCount : 1];
     __shared__ __align__(16) Real colStage[kWavesPerBlock][ColumnStage<Real, P>::kElems];
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
@@ -55,9 +55,9 @@ constexpr int kReduceThreads = kReduceWaves * kWave;
 // Last block: the likelihood, a fixed-order FP64 sum of the tile partials,
 // halved as in src/NewMultiDimensionalScaling.hpp:556, then scaled by
 // sumScale (1, or 0.5 tau for the untruncated value of :198-202).
-template <int P>
+template <typename PartT, int P>
 __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
-    const double* __restrict__ part, const double* __restrict__ tileSums, int numTiles,
+    const PartT* __restrict__ part, const double* __restrict__ tileSums, int numTiles,
     int nb, int n, int pOut, double* __restrict__ grad, double* __restrict__ sumOut, double sumScale) {
     __shared__ double red[kReduceThreads * (P > 1 ? P : 1)];
     const int tid = threadIdx.x;
@@ -90,15 +90,15 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
     double acc[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) acc[p] = 0.0;
-    const double* base = part + (int64_t)K * nb * (P * kTile) + lane;
+    const PartT* base = part + (int64_t)K * nb * (P * kTile) + lane;
     int J = wave;
     for (; J + (kAhead - 1) * kReduceWaves < nb; J += kAhead * kReduceWaves) {
         double v[kAhead][P];
 #pragma unroll
         for (int u = 0; u < kAhead; ++u) {
-            const double* slot = base + (int64_t)(J + u * kReduceWaves) * (P * kTile);
+            const PartT* slot = base + (int64_t)(J + u * kReduceWaves) * (P * kTile);
 #pragma unroll
-            for (int p = 0; p < P; ++p) v[u][p] = slot[p * kTile];
+            for (int p = 0; p < P; ++p) v[u][p] = (double)slot[p * kTile];
         }
 #pragma unroll
         for (int u = 0; u < kAhead; ++u)
@@ -106,9 +106,9 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
             for (int p = 0; p < P; ++p) acc[p] += v[u][p];
     }
     for (; J < nb; J += kReduceWaves) {
-        const double* slot = base + (int64_t)J * (P * kTile);
+        const PartT* slot = base + (int64_t)J * (P * kTile);
 #pragma unroll
-        for (int p = 0; p < P; ++p) acc[p] += slot[p * kTile];
+        for (int p = 0; p < P; ++p) acc[p] += (double)slot[p * kTile];
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) red[(p * kReduceWaves + wave) * kWave + lane] = acc[p];
@@ -130,7 +130,7 @@ static hipError_t go(const TileLaunch& a, hipStream_t s) {
     const dim3 grid((a.numTiles + kWavesPerBlock - 1) / kWavesPerBlock);
     hipLaunchKernelGGL((tile_kernel<Real, P, TRUNC, SYM, GRAD>), grid, dim3(kBlockThreads), 0, s,
                        (const Real*)a.delta, (const Real*)a.x, a.tiles, a.numTiles, a.nb, a.planeStride, a.tau,
-                       a.tileSums, a.part, a.devTau);
+                       a.tileSums, (Real*)a.part, a.devTau);
     return hipGetLastError();
 }
 
@@ -169,8 +169,12 @@ hipError_t launch_tiles_f32(const TileLaunch& a, hipStream_t s) { return dispatc
 template <int P>
 static hipError_t reduce_p(const ReduceLaunch& a, hipStream_t s) {
     const dim3 grid(a.gradient ? a.nb + 1 : 1);
-    hipLaunchKernelGGL((reduce_kernel<P>), grid, dim3(kReduceThreads), 0, s, a.part, a.tileSums,
-                       a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut, a.sumScale);
+    if (a.fp32)
+        hipLaunchKernelGGL((reduce_kernel<float, P>), grid, dim3(kReduceThreads), 0, s, (const float*)a.part,
+                           a.tileSums, a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut, a.sumScale);
+    else
+        hipLaunchKernelGGL((reduce_kernel<double, P>), grid, dim3(kReduceThreads), 0, s, (const double*)a.part,
+                           a.tileSums, a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut, a.sumScale);
     return hipGetLastError();
 }
 

@@ -28,8 +28,8 @@
 namespace mdship {
 
 // 64 MiB: the partials of P <= 2 instances up to N = 16,320 (FP64 slots
-// [nb][nb][2][64]), or of any P up to the N whose slots fit; the likelihood
-// alone needs 8 bytes per tile.
+// [nb][nb][2][64]; the FP32 engine's slots take half), or of any P up to the
+// N whose slots fit; the likelihood alone needs 8 bytes per tile.
 constexpr size_t kScratchBytes = (size_t)64 << 20;
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -95,7 +95,7 @@ void evaluate_noalloc(Engine& e, const double* locations, double* sumOut, double
     const bool gradient = gradientOut != nullptr;
     const size_t tileBytes = (size_t)e.numTiles * sizeof(double);
     const size_t partOffset = (tileBytes + 255) / 256 * 256;
-    const size_t partBytes = (size_t)e.nb * e.nb * e.kdim * kTile * sizeof(double);
+    const size_t partBytes = (size_t)e.nb * e.nb * e.kdim * kTile * e.realSize;
     const size_t need = gradient ? partOffset + partBytes : tileBytes;
     if (need > kScratchBytes)
         throw Error(MDS_HIP_ERR_ARGUMENT, "evaluate_device_noalloc: this instance's partials need " +
@@ -109,7 +109,7 @@ void evaluate_noalloc(Engine& e, const double* locations, double* sumOut, double
     if (locations) require_device_range(locations, rowMajorBytes, e.device, "locations");
     unsigned char* base = (unsigned char*)scratch_address(e.device);
     double* tileSums = (double*)base;
-    double* part = (double*)(base + partOffset);
+    void* part = base + partOffset;
     if (locations) {
         // the new locations replace the current ones, and a pending row with them
         e.rowWritePending = false;

@@ -183,12 +183,15 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
 // tileBase: this tile's plane-0 values; x: [nb * 64][P] locations; stage:
 // this wavefront's staged columns (see ColumnStage; unused for larger P);
 // writes *tileSum (lane 63) and the two gradient slots of part
-// ([nb][nb][P][64]).
+// ([nb][nb][P][64] in the engine's real type: each slot is the sum of one
+// 64 x 64 tile's terms for 64 rows, which the reduce adds in FP64; stored in
+// FP32 by the FP32 engine, whose per-term arithmetic already carries FP32
+// rounding, so that its partial traffic halves).
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, bool DIAG, int G>
 __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int64_t planeStride,
                                           const Real* __restrict__ x, const Real* __restrict__ stage, int I,
                                           int J, int nb, const PairConst<Real>& pc, double* __restrict__ tileSum,
-                                          double* __restrict__ part) {
+                                          Real* __restrict__ part) {
     static_assert(kTile % (2 * G) == 0 && (kTile / 2) % (2 * G) == 0, "group size must divide a quarter tile");
     constexpr bool kStaged = ColumnStage<Real, P>::kInLds;
     constexpr int kFirst = DIAG ? 1 : 0;                 // first step
@@ -272,19 +275,19 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
 
     if (GRAD) {
         // the accumulators hold half the gradient terms: doubled here (exact)
-        double* rowSlot = part + ((int64_t)I * nb + J) * (P * kTile) + lane;
+        Real* rowSlot = part + ((int64_t)I * nb + J) * (P * kTile) + lane;
         if constexpr (DIAG) {
             // after the walk lane l holds column (l + kFirst + kSteps) mod 64
             constexpr int kAlign = (2 * kTile - kFirst - kSteps) % kTile;
 #pragma unroll
             for (int p = 0; p < P; ++p)
-                rowSlot[p * kTile] = 2.0 * (acc.row[p].value() + lane_rotate<kAlign>(acc.col[p].value()));
+                rowSlot[p * kTile] = (Real)(2.0 * (acc.row[p].value() + lane_rotate<kAlign>(acc.col[p].value())));
         } else {
-            double* colSlot = part + ((int64_t)J * nb + I) * (P * kTile) + lane;
+            Real* colSlot = part + ((int64_t)J * nb + I) * (P * kTile) + lane;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                rowSlot[p * kTile] = 2.0 * acc.row[p].value();
-                colSlot[p * kTile] = 2.0 * acc.col[p].value();
+                rowSlot[p * kTile] = (Real)(2.0 * acc.row[p].value());
+                colSlot[p * kTile] = (Real)(2.0 * acc.col[p].value());
             }
         }
     }
@@ -295,7 +298,7 @@ template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, int G>
 __device__ __forceinline__ void tile_walk(const Real* __restrict__ tileBase, int64_t planeStride,
                                           const Real* __restrict__ x, const Real* __restrict__ stage, int I,
                                           int J, int nb, const PairConst<Real>& pc, double* __restrict__ tileSum,
-                                          double* __restrict__ part) {
+                                          Real* __restrict__ part) {
     if (I == J)
         tile_body<Real, P, TRUNC, SYM, GRAD, true, G>(tileBase, planeStride, x, stage, I, J, nb, pc, tileSum, part);
     else
