@@ -342,22 +342,80 @@ def _bench_batch(steps, warmup, count=64, tune_cache=None, no_cpu=False, cpu_bud
 def _bench_hmc(iterations=2000, burn_in=500, n=5000, p=2):
     """Plain HMC on the GPU: N = 5000, P = 2, FP64, euclid10 with 2% missing,
     truncation, 20 leapfrog steps over trajectory 0.1, tau and the trait
-    precision learned, each trajectory one hipGraph launch."""
+    precision learned, each trajectory one hipGraph launch.  Two runs: the
+    precision updated by the reference's random walk of fixed width 0.01
+    (R/Untitled.R:668-691), and with that width adapted during burn-in."""
     from massivemds_amd import datagen, hmc
 
     delta = datagen.euclid10(n, missing=0.02)
-    res = hmc.hmcsampler(iterations, burnIn=burn_in, data=delta, latentDimension=p, learnPrec=True,
-                         learnTraitPrec=True, trajectory=0.1, mdsPrecision=1.0, truncation=True, seed=666)
     kept = iterations - burn_in
+    out = {"workload": f"hmc_n{n}_p{p}_fp64_trunc_missing2", "iterations": iterations, "burn_in": burn_in}
+    for name, adapt in (("reference_width", False), ("adapted_width", True)):
+        print(f"hmc5000: {name}", file=sys.stderr, flush=True)
+        res = hmc.hmcsampler(iterations, burnIn=burn_in, data=delta, latentDimension=p, learnPrec=True,
+                             learnTraitPrec=True, trajectory=0.1, mdsPrecision=1.0, truncation=True, seed=666,
+                             adaptPrecisionWidth=adapt)
+        out[name] = {
+            "accept_rate": res["acprat"], "precision_accept_rate": res["precision_acceptance"],
+            "precision_width": res["precision_width"],
+            "sampling_seconds": res["Time"], "ms_per_iteration": res["Time"] / kept * 1e3,
+            "min_ess": res["min_ess"], "min_ess_per_s": res["min_ess_per_s"],
+            "ess": {k: round(v, 1) for k, v in res["ess"].items()},
+            "posterior_mean_precision": float(np.mean(res["precision"])),
+            "evaluations": res["evaluations"], "total_seconds": res["total_seconds"],
+        }
+    print(json.dumps(out))
+
+
+def _bench_hmc_chains(chains=4, iterations=2000, burn_in=500, n=5000, p=2):
+    """Many chains in one launch per leapfrog step: `chains` HMC chains of
+    the N = 5000 workload (shared observations, seeds 666 + k, adapted
+    precision width), each trajectory of every chain one hipGraph launch,
+    against one chain of the single-chain sampler on the same data.
+    Reports time per iteration of all chains and the summed min-ESS/s."""
+    from massivemds_amd import datagen, hmc
+    from massivemds_amd.batch import MultiDimensionalScalingBatch
+    from massivemds_amd.engine import Flags
+
+    delta = datagen.euclid10(n, missing=0.02)
+    kept = iterations - burn_in
+    kw = dict(iterations=iterations, burnIn=burn_in, leapfrogSteps=20, trajectory=0.1, learnPrecision=1,
+              learnTraitPrecision=1, adaptPrecisionWidth=1)
+    print("hmc5000 chains: single chain", file=sys.stderr, flush=True)
+    single = hmc.hmcsampler(iterations, burnIn=burn_in, data=delta, latentDimension=p, learnPrec=True,
+                            learnTraitPrec=True, trajectory=0.1, mdsPrecision=1.0, truncation=True, seed=666,
+                            adaptPrecisionWidth=True)
+    flags = Flags.OPENCL | Flags.LEFT_TRUNCATION
+    b = MultiDimensionalScalingBatch([p] * chains, [n] * chains, flags, 0)
+    b.setPairwiseData(0, delta)
+    for k in range(1, chains):
+        b.sharePairwiseData(k, 0)
+    for k in range(chains):
+        b.setParameters(k, [1.0])
+    x0 = datagen.normals(666, 3, n * p).reshape(n, p)  # the single-chain sampler's default start
+    print(f"hmc5000 chains: {chains} chains in one launch", file=sys.stderr, flush=True)
+    traces, _, stats = b.run_hmc([x0] * chains, hmc.default_options(seed=666, **kw))
+    b.close()
+    cols = [c for c in hmc.TRACE_COLUMNS]
+    per_chain = []
+    for k in range(chains):
+        ess = {c: hmc.effective_sample_size(traces[k][:, j]) for j, c in enumerate(cols)}
+        finite = [v for v in ess.values() if np.isfinite(v)]
+        per_chain.append({"accept_rate": stats[k].acceptRate, "precision_accept_rate": stats[k].precisionAcceptRate,
+                          "min_ess": min(finite), "ess": {c: round(v, 1) for c, v in ess.items()}})
+    seconds = stats[0].samplingSeconds
+    total_min_ess = sum(c["min_ess"] for c in per_chain)
     out = {
-        "workload": f"hmc_n{n}_p{p}_fp64_trunc_missing2", "iterations": iterations, "burn_in": burn_in,
-        "accept_rate": res["acprat"], "precision_accept_rate": res["precision_acceptance"],
-        "sampling_seconds": res["Time"], "ms_per_iteration": res["Time"] / kept * 1e3,
-        "min_ess": res["min_ess"], "min_ess_per_s": res["min_ess_per_s"],
-        "ess": {k: round(v, 1) for k, v in res["ess"].items()},
-        "posterior_mean_precision": float(np.mean(res["precision"])),
-        "evaluations": res["evaluations"], "total_seconds": res["total_seconds"],
+        "workload": f"hmc_n{n}_p{p}_fp64_trunc_missing2_chains{chains}", "chains": chains,
+        "iterations": iterations, "burn_in": burn_in,
+        "batched": {"sampling_seconds": seconds, "ms_per_iteration_all_chains": seconds / kept * 1e3,
+                    "summed_min_ess": total_min_ess, "summed_min_ess_per_s": total_min_ess / seconds,
+                    "chains": per_chain},
+        "single_chain": {"sampling_seconds": single["Time"], "ms_per_iteration": single["Time"] / kept * 1e3,
+                         "min_ess": single["min_ess"], "min_ess_per_s": single["min_ess_per_s"]},
     }
+    out["throughput_ratio"] = (chains * single["Time"]) / seconds
+    out["min_ess_per_s_ratio"] = out["batched"]["summed_min_ess_per_s"] / single["min_ess_per_s"]
     print(json.dumps(out))
 
 
@@ -379,27 +437,32 @@ def _bench_leapfrog(steps, warmup, settle_ms, n=5000, p=2, tau=1.0):
         "tile_kernel_us": r["tile_ms"] * 1e3, "eval_us_events": r["eval_ms"] * 1e3, "tiles": r["tiles"]}))
 
 
-def _bench_surrogate(iterations=500, n=20000, p=5):
+def _bench_surrogate(iterations=8000, n=20000, p=5, sweep_iterations=2000):
     """Surrogate-trajectory HMC against plain HMC: N = 20000, P = 5, FP64,
     euclid10 with 2% missing, truncation, precision fixed at its plug-in value
     at the posterior mode, trait precision I.  Warm-up: 100 plain iterations
     of 20 leapfrog steps (2000 exact gradient evaluations, all stored as
     training snapshots) with dual-averaged step size; training: 4 epochs over
     the snapshots on the GPU; then `iterations` of each sampler from the same
-    state and step size.  Reports min-ESS/s of both, their ratio and the
-    largest |difference| of posterior means in Monte Carlo standard errors."""
+    state and step size (the surrogate one with its exact-gradient
+    fallback), and the surrogate chain tuned on its own for L = 2, 5, 10, 20
+    (dual-averaged step, no fallback, `sweep_iterations` each).  Reports
+    min-ESS/s of every run, the ratios to plain HMC and the largest
+    |difference| of posterior means in Monte Carlo standard errors."""
     from massivemds_amd import datagen
     from massivemds_amd.surrogate import compare_samplers
 
     delta = datagen.euclid10(n, missing=0.02)
-    r = compare_samplers(delta, p, iterations, warmup_evals=2000, leapfrog=20, truncation=True, epochs=4)
+    r = compare_samplers(delta, p, iterations, warmup_evals=2000, leapfrog=20, truncation=True, epochs=4,
+                         sweep=(2, 5, 10, 20), sweep_iterations=sweep_iterations, grid=True,
+                         log=lambda m: print(f"surrogate20000: {m}", file=sys.stderr, flush=True))
     del delta
-    runs = {}
-    for name in ("plain", "surrogate"):
-        v = r[name]
-        runs[name] = {k: (round(float(v[k]), 4) if isinstance(v[k], float) else v[k])
-                      for k in ("accept_rate", "seconds", "ms_per_iteration", "min_ess", "min_ess_per_s",
-                                "surrogate_iterations", "exact_iterations", "surrogate_accept_rate")}
+    keys = ("accept_rate", "seconds", "ms_per_iteration", "min_ess", "min_ess_per_s", "surrogate_iterations",
+            "exact_iterations", "surrogate_accept_rate", "step_size")
+
+    def brief(v):
+        return {k: (round(float(v[k]), 6) if isinstance(v[k], float) else v[k]) for k in keys}
+
     out = {
         "metric": "min-ESS/s surrogate vs plain HMC", "value": round(r["ess_per_s_ratio"], 4), "unit": "ratio",
         "higher_is_better": True, "dtype": "f64 likelihood, bf16 network",
@@ -408,7 +471,16 @@ def _bench_surrogate(iterations=500, n=20000, p=5):
         "precision": r["precision"], "step_size": r["step_size"], "warmup_accept": r["warmup_accept"],
         "snapshots": r["snapshots"], "train_loss": r["train_loss"], "map_s": r["map_s"],
         "warmup_s": r["warmup_s"], "train_s": r["train_s"], "max_mean_z": r["max_mean_z"],
-        "plain": runs["plain"], "surrogate": runs["surrogate"],
+        "plain": brief(r["plain"]), "surrogate": brief(r["surrogate"]),
+        "plain_ess": [round(float(v), 1) for v in r["plain"]["ess"]],
+        "tuned": [dict(brief(t), leapfrog=t["leapfrog"], max_mean_z=round(t["max_mean_z"], 2)) for t in r["tuned"]],
+        "grid": [{"leapfrog": g["leapfrog"], "step_size": g["step_size"], "trajectory": g["trajectory"],
+                  "accept_rate": round(g["accept_rate"], 4), "ms_per_iteration": round(g["ms_per_iteration"], 4)}
+                 for g in r["grid"]],
+        "note": "best_tuned_ratio counts only tuned runs whose posterior means match the plain chain's within "
+                "2 Monte Carlo SE (null: none did)",
+        "best_tuned_ratio": r["best_tuned_ratio"], "best_tuned_z": r["best_tuned_z"],
+        "columns": r["columns"],
     }
     print(json.dumps(out))
 
@@ -620,15 +692,18 @@ def main():
                     help="locations for the default workload (the metric is quoted at 10000; other sizes "
                          "for scaling studies)")
     ap.add_argument("--workload", default="n10000",
-                    choices=["n10000", "n1000", "batch64", "hmc5000", "leapfrog5000", "surrogate20000",
-                             "rowupdate10000", "beast372"],
+                    choices=["n10000", "n1000", "batch64", "hmc5000", "hmc5000x4", "leapfrog5000",
+                             "surrogate20000", "rowupdate10000", "beast372"],
                     help="n10000: the metric's workload (default); n1000: the reference benchmark's loop "
                          "at N = 1000 (fresh locations per iteration, through the API); batch64: the uneven batch; "
-                         "hmc5000: plain HMC; leapfrog5000: the HMC workload's per-step gradient pass, launched "
+                         "hmc5000: plain HMC; hmc5000x4: four HMC chains in one launch per leapfrog step; "
+                         "leapfrog5000: the HMC workload's per-step gradient pass, launched "
                          "directly (for kernel traces); surrogate20000: surrogate against plain HMC; "
                          "rowupdate10000: single-location moves through the O(N) update; "
                          "beast372: real BEAST data (N = 372) against the logged likelihood")
-    ap.add_argument("--iterations", type=int, default=500, help="sampling iterations (surrogate20000)")
+    ap.add_argument("--iterations", type=int, default=8000, help="sampling iterations (surrogate20000)")
+    ap.add_argument("--sweep-iterations", type=int, default=2000,
+                    help="sampling iterations per tuned surrogate run (surrogate20000)")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch64)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to scale "
@@ -650,11 +725,14 @@ def main():
     if args.workload == "hmc5000":
         _bench_hmc()
         return
+    if args.workload == "hmc5000x4":
+        _bench_hmc_chains()
+        return
     if args.workload == "leapfrog5000":
         _bench_leapfrog(args.steps, args.warmup, args.settle_ms)
         return
     if args.workload == "surrogate20000":
-        _bench_surrogate(args.iterations)
+        _bench_surrogate(args.iterations, sweep_iterations=args.sweep_iterations)
         return
     if args.workload == "rowupdate10000":
         _bench_row_updates()

@@ -268,13 +268,95 @@ def plugin_precision(delta, x) -> float:
     return float(0.5 * obs.sum() / (2.0 * s))
 
 
+def _chain_summary(trace, st, iterations):
+    """Accept rate, time, min-ESS and min-ESS/s of a fixed-precision run over
+    the invariant functionals (log posterior, distances, norms; a chain that
+    never moved has ESS 0)."""
+    from . import hmc
+
+    cols = [0] + list(range(3, len(hmc.TRACE_COLUMNS)))  # tau and trace(T) are fixed here
+    ess = np.array([hmc.effective_sample_size(trace[:, k]) for k in cols])
+    ess = np.where(np.isfinite(ess), ess, 0.0)
+    means = trace[:, cols].mean(axis=0)
+    se = trace[:, cols].std(axis=0) / np.sqrt(np.maximum(ess, 1.0))
+    return {
+        "accept_rate": st.acceptRate, "seconds": st.samplingSeconds,
+        "ms_per_iteration": st.samplingSeconds / iterations * 1e3, "min_ess": float(ess.min()),
+        "min_ess_per_s": float(ess.min()) / st.samplingSeconds if st.samplingSeconds > 0 else 0.0,
+        "surrogate_iterations": st.surrogateIterations, "exact_iterations": st.exactIterations,
+        "surrogate_accept_rate": st.surrogateAcceptRate, "step_size": st.stepSize,
+        "means": means, "se": se, "ess": ess,
+    }
+
+
+def sweep_surrogate(eng, sur, x_start, precision, step0: float, leapfrogs=(2, 5, 10, 20), burn_in: int = 200,
+                    iterations: int = 2000, target_accept: float = 0.65, seed: int = 900):
+    """The surrogate chain tuned on its own: for each leapfrog count L, a
+    burn-in of surrogate trajectories dual-averages the chain's own step size,
+    from ``step0``, toward ``target_accept`` (no exact-gradient fallback, so only surrogate
+    trajectories are proposed), then ``iterations`` more are sampled at the
+    adapted step.  Returns one summary per L (acceptance, step size, time per
+    iteration, min-ESS and min-ESS/s over the invariant functionals)."""
+    from . import hmc
+
+    out = []
+    for k, L in enumerate(leapfrogs):
+        eng.setParameters([precision])
+        o = hmc.default_options(iterations=burn_in + iterations, burnIn=burn_in, leapfrogSteps=int(L),
+                                trajectory=step0 * L, learnPrecision=0, learnTraitPrecision=0, seed=seed + k,
+                                adaptStepSize=1, targetAccept=target_accept, surrogate=sur.handle, useSurrogate=1,
+                                fallbackWindow=0)
+        trace, _, st = hmc.run_hmc(eng, x_start, o)
+        r = _chain_summary(trace, st, iterations)
+        r["leapfrog"] = int(L)
+        out.append(r)
+    return out
+
+
+def grid_surrogate(eng, sur, x_start, precision, step0: float, leapfrogs=(2, 5, 10, 20),
+                   fractions=(1.0, 0.25, 0.0625, 0.015625), iterations: int = 400, seed: int = 950):
+    """Acceptance of surrogate trajectories over a grid of fixed step sizes
+    (``fractions`` of ``step0``) and leapfrog counts, with no exact-gradient
+    fallback: where the gradient error, not the integrator, sets the
+    acceptance, the acceptance follows the trajectory length (step x L)
+    whatever the split.  The runs are short: their ESS figures are not
+    meaningful, only their acceptance."""
+    from . import hmc
+
+    out = []
+    k = 0
+    for L in leapfrogs:
+        for f in fractions:
+            eng.setParameters([precision])
+            o = hmc.default_options(iterations=iterations, burnIn=0, leapfrogSteps=int(L), stepSize=step0 * f,
+                                    learnPrecision=0, learnTraitPrecision=0, seed=seed + k, surrogate=sur.handle,
+                                    useSurrogate=1, fallbackWindow=0)
+            trace, _, st = hmc.run_hmc(eng, x_start, o)
+            r = _chain_summary(trace, st, iterations)
+            r["leapfrog"] = int(L)
+            r["trajectory"] = step0 * f * L
+            out.append(r)
+            k += 1
+    return out
+
+
 def compare_samplers(delta, p: int, iterations: int, warmup_evals: int = 2000, leapfrog: int = 20,
                      truncation: bool = True, precision=None, epochs: int = 4, learning_rate: float = 2e-3,
-                     fallback_rate: float = 0.2, fallback_window: int = 20, seed: int = 666, map_steps: int = 300):
+                     fallback_rate: float = 0.2, fallback_window: int = 20, seed: int = 666, map_steps: int = 300,
+                     sweep=None, sweep_iterations: int = 2000, grid=False, log=None):
     """Plain-gradient HMC against surrogate-trajectory HMC on the same target
-    (fixed precision and trait precision), from the same warm start and step
-    size.  Returns a dict with both runs' statistics and the min-ESS/s ratio."""
+    (fixed precision and trait precision), from the same warm start: the
+    plain chain at the step size dual-averaged during the warm-up, the
+    surrogate chain first at the same step and leapfrog count (with the
+    exact-gradient fallback), then, with ``sweep`` (leapfrog counts), tuned on
+    its own by ``sweep_surrogate``.  Returns a dict with the runs' statistics,
+    the min-ESS/s ratio of the fallback run and of the best tuned run, and
+    the largest difference of posterior means in Monte Carlo standard
+    errors.  ``log``, when given, receives a line at the start of each
+    stage."""
     from . import datagen, hmc
+
+    say = log or (lambda message: None)
     from .engine import Flags, MultiDimensionalScaling
 
     delta = np.asarray(delta, dtype=np.float64)
@@ -286,6 +368,7 @@ def compare_samplers(delta, p: int, iterations: int, warmup_evals: int = 2000, l
     import time
 
     t0 = time.perf_counter()
+    say("posterior mode")
     x = map_locations(eng, datagen.normals(seed, 3, n * p).reshape(n, p), steps=map_steps)
     if precision is None:
         precision = plugin_precision(delta, x)
@@ -299,43 +382,55 @@ def compare_samplers(delta, p: int, iterations: int, warmup_evals: int = 2000, l
                               trajectory=0.01 * leapfrog, learnPrecision=0, learnTraitPrecision=0, seed=seed,
                               adaptStepSize=1, targetAccept=0.8, surrogate=sur.handle, collectSnapshots=1)
     t0 = time.perf_counter()
+    say("warm-up with snapshots")
     _, x_warm, st_warm = hmc.run_hmc(eng, x, opt)
     t_warm = time.perf_counter() - t0
     eps = st_warm.stepSize
 
     t0 = time.perf_counter()
+    say("training")
     loss = sur.train(epochs, learning_rate)
     t_train = time.perf_counter() - t0
 
     runs = {}
     for name, use, chain_seed in (("plain", 0, seed + 1), ("surrogate", 1, seed + 2)):  # independent chains
         eng.setParameters([precision])
+        say(f"{name} chain")
         o = hmc.default_options(iterations=iterations, burnIn=0, leapfrogSteps=leapfrog, stepSize=eps,
                                 learnPrecision=0, learnTraitPrecision=0, seed=chain_seed, surrogate=sur.handle,
                                 useSurrogate=use, fallbackAcceptRate=fallback_rate,
                                 fallbackWindow=fallback_window)
         trace, _, st = hmc.run_hmc(eng, x_warm, o)
-        cols = [0] + list(range(3, len(hmc.TRACE_COLUMNS)))  # tau and trace(T) are fixed here
-        ess = np.array([hmc.effective_sample_size(trace[:, k]) for k in cols])
-        means = trace[:, cols].mean(axis=0)
-        se = trace[:, cols].std(axis=0) / np.sqrt(np.maximum(ess, 1.0))
-        runs[name] = {
-            "accept_rate": st.acceptRate, "seconds": st.samplingSeconds,
-            "ms_per_iteration": st.samplingSeconds / iterations * 1e3, "min_ess": float(np.nanmin(ess)),
-            "min_ess_per_s": float(np.nanmin(ess)) / st.samplingSeconds,
-            "surrogate_iterations": st.surrogateIterations, "exact_iterations": st.exactIterations,
-            "surrogate_accept_rate": st.surrogateAcceptRate,
-            "means": means, "se": se, "ess": ess,
-        }
-    z = np.abs(runs["plain"]["means"] - runs["surrogate"]["means"]) / np.hypot(runs["plain"]["se"],
-                                                                              runs["surrogate"]["se"])
+        runs[name] = _chain_summary(trace, st, iterations)
+
+    def zmax(a, b):
+        z = np.abs(a["means"] - b["means"]) / np.hypot(a["se"], b["se"])
+        return float(np.max(z))
+
+    say("tuned surrogate chains")
+    tuned = sweep_surrogate(eng, sur, x_warm, precision, eps, sweep, iterations=sweep_iterations) if sweep else []
+    for r in tuned:
+        r["max_mean_z"] = zmax(runs["plain"], r)
+    # a rate counts only where the chain's means match the plain chain's
+    # within 2 Monte Carlo standard errors: a chain that has not mixed has
+    # no meaningful ESS
+    matched = [r for r in tuned if r["max_mean_z"] <= 2.0]
+    best = max(matched, key=lambda r: r["min_ess_per_s"]) if matched else None
+    if grid:
+        say("fixed-step grid")
+    grid_runs = grid_surrogate(eng, sur, x_warm, precision, eps) if grid else []
     eng.close()
     sur.close()
+    plain_rate = runs["plain"]["min_ess_per_s"]
     return {
         "n": n, "p": p, "precision": precision, "step_size": eps, "warmup_accept": st_warm.acceptRate,
         "snapshots": st_warm.snapshots, "train_loss": loss, "map_s": t_map, "warmup_s": t_warm,
         "train_s": t_train, "plain": runs["plain"], "surrogate": runs["surrogate"],
-        "ess_per_s_ratio": runs["surrogate"]["min_ess_per_s"] / runs["plain"]["min_ess_per_s"],
-        "max_mean_z": float(np.max(z)), "columns": [hmc.TRACE_COLUMNS[k] for k in
-                                                     [0] + list(range(3, len(hmc.TRACE_COLUMNS)))],
+        "ess_per_s_ratio": runs["surrogate"]["min_ess_per_s"] / plain_rate,
+        "max_mean_z": zmax(runs["plain"], runs["surrogate"]),
+        "tuned": tuned,
+        "grid": grid_runs,
+        "best_tuned_ratio": best["min_ess_per_s"] / plain_rate if best else None,
+        "best_tuned_z": best["max_mean_z"] if best else None,
+        "columns": [hmc.TRACE_COLUMNS[k] for k in [0] + list(range(3, len(hmc.TRACE_COLUMNS)))],
     }

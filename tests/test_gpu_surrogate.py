@@ -184,3 +184,119 @@ def test_surrogate_chain_matches_plain_posterior():
     assert np.all(z < 3.5)
     eng.close()
     sur.close()
+
+
+def _log_posterior(eng, x, precision, truncation):
+    """Exact log posterior of the sampler's model at x (trait precision I,
+    row covariance I_N): the engine's log-likelihood (R/mds.R:24-28) plus the
+    matrix-normal log prior (R/Untitled.R:68-95)."""
+    n, p = x.shape
+    eng.updateLocations(-1, x)
+    s = eng.getSumOfIncrements()
+    loglik = 0.5 * (np.log(precision) - np.log(2.0 * np.pi)) * n * (n - 1) / 2.0 - s
+    return loglik - 0.5 * float(np.sum(x * x)) - 0.5 * n * p * np.log(2.0 * np.pi)
+
+
+@pytest.mark.timeout(900)
+def test_config5_surrogate_sampler_at_full_size():
+    """The surrogate-trajectory sampler at its workload's shape: N = 20,000,
+    P = 5, distances of N(0, I_10) points with 2% missing, truncation, the
+    5 + 16 -> 256 -> 256 -> 5 network, 2,000 warm-up evaluations.  Checked
+    through properties that hold at any size:
+      * the warm-up stores 1 + L snapshots per accepted trajectory (a
+        rejected trajectory's snapshots are dropped);
+      * the fallback schedule: with every window failing (threshold above 1)
+        windows of W surrogate iterations alternate with exact windows of
+        W, 2W, 4W iterations;
+      * the accept step judges with the exact likelihood: after runs that
+        accept surrogate trajectories, the log posterior the trace records
+        for the returned state equals a fresh exact evaluation there;
+      * every trace entry is finite."""
+    n, p, L = 20000, 5, 20
+    print(f"config 5: N={n} P={p}: data", flush=True)
+    delta = datagen.euclid10(n, missing=0.02)
+    eng = MultiDimensionalScaling(p, n, Flags.OPENCL | Flags.LEFT_TRUNCATION, 0)
+    eng.setPairwiseData(delta)
+    eng.setParameters([1.0])
+    x = map_locations(eng, datagen.normals(666, 3, n * p).reshape(n, p), steps=60)
+    tau = plugin_precision(delta, x)
+    del delta
+    eng.setParameters([tau])
+    print(f"config 5: precision {tau:.4f}; warm-up", flush=True)
+    warm_evals = 2000
+    sur = Surrogate(n, p, warm_evals + L + 1, seed=5)
+    its = warm_evals // L
+    opt = hmc.default_options(iterations=its + 1, burnIn=its, leapfrogSteps=L, trajectory=0.01 * L,
+                              learnPrecision=0, learnTraitPrecision=0, seed=7, adaptStepSize=1, targetAccept=0.8,
+                              surrogate=sur.handle, collectSnapshots=1)
+    trace, xw, st = hmc.run_hmc(eng, x, opt)
+    accepted = round(st.acceptRate * (its + 1))
+    assert st.snapshots == 1 + L * accepted and sur.snapshot_count() == st.snapshots
+    assert accepted > 0 and np.all(np.isfinite(trace))
+    loss = sur.train(4, 2e-3)
+    assert np.isfinite(loss)
+    eps = st.stepSize
+    # fallback schedule: every window fails
+    w = 5
+    o = hmc.default_options(iterations=10 * w, burnIn=0, leapfrogSteps=L, stepSize=eps, learnPrecision=0,
+                            learnTraitPrecision=0, seed=8, surrogate=sur.handle, useSurrogate=1,
+                            fallbackAcceptRate=1.01, fallbackWindow=w)
+    trace, xf, s = hmc.run_hmc(eng, xw, o)
+    print(f"config 5: fallback schedule surrogate {s.surrogateIterations:.0f} exact {s.exactIterations:.0f}",
+          flush=True)
+    assert (s.surrogateIterations, s.exactIterations) == (3 * w, 7 * w)
+    assert np.all(np.isfinite(trace))
+    # accepted surrogate trajectories (short, small steps), exact energies of the returned states
+    cur, surrogate_accepts = xf, 0
+    for k in range(6):
+        o = hmc.default_options(iterations=2, burnIn=0, leapfrogSteps=2, stepSize=0.05 * eps, learnPrecision=0,
+                                learnTraitPrecision=0, seed=20 + k, surrogate=sur.handle, useSurrogate=1,
+                                fallbackWindow=0)
+        trace, cur, s = hmc.run_hmc(eng, cur, o)
+        assert np.all(np.isfinite(trace))
+        surrogate_accepts += round(s.surrogateAcceptRate * s.surrogateIterations)
+        exact = _log_posterior(eng, cur, tau, True)
+        assert abs(trace[-1, 0] - exact) <= 1e-12 * abs(exact), (k, trace[-1, 0], exact)
+    print(f"config 5: accepted surrogate trajectories {surrogate_accepts} of 12", flush=True)
+    assert surrogate_accepts > 0
+    eng.close()
+    sur.close()
+
+
+@pytest.mark.timeout(900)
+def test_surrogate_chain_matches_plain_posterior_p5():
+    """Exactness at P = 5: on a posterior where two independent plain chains
+    agree (distances of 5-D points, N = 120, so the model is well specified),
+    the surrogate chain's invariant functionals agree with them within
+    Monte Carlo error."""
+    n, p, L, its = 120, 5, 10, 6000
+    delta, eng, x, _ = _problem(n, p, 41, True, dim=5, noise=0.3)
+    sur = Surrogate(n, p, 30 * L + L + 1, seed=4)
+    opt = hmc.default_options(iterations=31, burnIn=30, leapfrogSteps=L, trajectory=0.02 * L, learnPrecision=0,
+                              learnTraitPrecision=0, seed=6, adaptStepSize=1, surrogate=sur.handle,
+                              collectSnapshots=1)
+    _, xw, st = hmc.run_hmc(eng, x, opt)
+    sur.train(6, 2e-3)
+    cols = [0] + list(range(3, len(hmc.TRACE_COLUMNS)))
+    res = {}
+    for name, use, seed in (("plain", 0, 8), ("plain2", 0, 9), ("surrogate", 1, 10)):
+        o = hmc.default_options(iterations=its, burnIn=0, leapfrogSteps=L, stepSize=st.stepSize, learnPrecision=0,
+                                learnTraitPrecision=0, seed=seed, surrogate=sur.handle, useSurrogate=use,
+                                fallbackAcceptRate=0.2, fallbackWindow=10)
+        trace, _, s = hmc.run_hmc(eng, xw, o)
+        t = trace[its // 10:, cols]
+        ess = np.array([hmc.effective_sample_size(t[:, k]) for k in range(t.shape[1])])
+        res[name] = (t.mean(axis=0), t.std(axis=0) / np.sqrt(np.maximum(ess, 1.0)), s)
+        print(f"{name}: accept {s.acceptRate:.3f} surrogate its {s.surrogateIterations:.0f} min ESS "
+              f"{np.nanmin(ess):.0f}", flush=True)
+    m0, se0, _ = res["plain"]
+    m2, se2, _ = res["plain2"]
+    m1, se1, s1 = res["surrogate"]
+    z_plain = np.abs(m0 - m2) / np.hypot(se0, se2)
+    z = np.abs(m0 - m1) / np.hypot(se0, se1)
+    print("z plain-plain", np.round(z_plain, 2), "z plain-surrogate", np.round(z, 2), flush=True)
+    assert np.all(z_plain < 3.5)  # the two plain chains agree: the comparison is meaningful
+    assert s1.surrogateIterations > 0
+    assert np.all(z < 3.5)
+    eng.close()
+    sur.close()
