@@ -704,6 +704,9 @@ def main():
     ap.add_argument("--iterations", type=int, default=8000, help="sampling iterations (surrogate20000)")
     ap.add_argument("--sweep-iterations", type=int, default=2000,
                     help="sampling iterations per tuned surrogate run (surrogate20000)")
+    ap.add_argument("--hmc-iterations", type=int, default=2000,
+                    help="iterations of the HMC workloads (hmc5000, hmc5000x4; the workload is quoted at 2000)")
+    ap.add_argument("--hmc-burn-in", type=int, default=500, help="burn-in of the HMC workloads")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (batch64)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (read + write), to scale "
@@ -723,10 +726,10 @@ def main():
         print(json.dumps(_bench_batch(args.steps, args.warmup, tune_cache=args.tune_cache, no_cpu=args.no_cpu)))
         return
     if args.workload == "hmc5000":
-        _bench_hmc()
+        _bench_hmc(args.hmc_iterations, args.hmc_burn_in)
         return
     if args.workload == "hmc5000x4":
-        _bench_hmc_chains()
+        _bench_hmc_chains(iterations=args.hmc_iterations, burn_in=args.hmc_burn_in)
         return
     if args.workload == "leapfrog5000":
         _bench_leapfrog(args.steps, args.warmup, args.settle_ms)
