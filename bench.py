@@ -364,7 +364,7 @@ def _bench_hmc(iterations=2000, burn_in=500, n=5000, p=2):
             "posterior_mean_precision": float(np.mean(res["precision"])),
             "evaluations": res["evaluations"], "total_seconds": res["total_seconds"],
         }
-    print(json.dumps(out))
+    return out
 
 
 def _bench_hmc_chains(chains=4, iterations=2000, burn_in=500, n=5000, p=2):
@@ -686,8 +686,8 @@ def main():
                          "settles under sustained load), before each timed pass of the default and leapfrog5000 "
                          "workloads; 0: none")
     ap.add_argument("--no-extra", action="store_true",
-                    help="default workload only: skip the reference loop at N = 1000 and the uneven batch that "
-                         "the line carries beside it")
+                    help="default workload only: skip the reference loop at N = 1000, the uneven batch and the "
+                         "HMC sampler at N = 5000 that the line carries beside it")
     ap.add_argument("--n", type=int, default=10000,
                     help="locations for the default workload (the metric is quoted at 10000; other sizes "
                          "for scaling studies)")
@@ -726,7 +726,7 @@ def main():
         print(json.dumps(_bench_batch(args.steps, args.warmup, tune_cache=args.tune_cache, no_cpu=args.no_cpu)))
         return
     if args.workload == "hmc5000":
-        _bench_hmc(args.hmc_iterations, args.hmc_burn_in)
+        print(json.dumps(_bench_hmc(args.hmc_iterations, args.hmc_burn_in)))
         return
     if args.workload == "hmc5000x4":
         _bench_hmc_chains(iterations=args.hmc_iterations, burn_in=args.hmc_burn_in)
@@ -817,6 +817,7 @@ def main():
             "reference_loop_n1000": _bench_reference_loop(no_cpu=args.no_cpu, cpu_budget=min(args.cpu_budget, 10.0)),
             "batch64_uneven_fp32": _bench_batch(20, 3, no_cpu=args.no_cpu,
                                                 cpu_budget=min(args.cpu_budget, 10.0)),
+            "hmc_n5000": _bench_hmc(),
         }
     print(json.dumps(out))
 

@@ -116,16 +116,23 @@ __device__ __forceinline__ void batch_tile_any(const BatchDataset& d, uint32_t t
 
 // KDIM = 0: items of any embedding dimension (switch per item); KDIM > 0:
 // every item has that kernel dimension.
+// Staged column elements per wavefront of a launch (see ColumnStage).
+template <typename Real, int KDIM>
+constexpr int batch_stage_elems() {
+    return KDIM == 0 ? ColumnStage<Real, 1>::kMaxElems : ColumnStage<Real, KDIM>::kElems;
+}
+
 template <typename Real, int KDIM>
 __global__ __launch_bounds__(kBatchMaxThreads) void batch_tile_kernel(const BatchDataset* __restrict__ ds,
                                                                        const BatchItem* __restrict__ items,
                                                                        int numItems,
                                                                        double* __restrict__ tileSums) {
     __shared__ __align__(16) Real lnPhiTab[TableOf<Real>::kCount];  // truncated datasets read it
-    __shared__ __align__(16) Real colStage[kBatchMaxThreads / kWave][ColumnStage<Real, 1>::kMaxElems];
+    // staged columns: batch_stage_elems values per wavefront, sized by the launch
+    extern __shared__ __align__(16) unsigned char batchColStage[];
     copy_lnphi_table(lnPhiTab);
     __syncthreads();
-    Real* stage = colStage[threadIdx.x / kWave];
+    Real* stage = (Real*)batchColStage + (threadIdx.x / kWave) * batch_stage_elems<Real, KDIM>();
     const int wavesPerBlock = blockDim.x / kWave;
     const int stride = gridDim.x * wavesPerBlock;
     for (int w = __builtin_amdgcn_readfirstlane(blockIdx.x * wavesPerBlock + threadIdx.x / kWave); w < numItems;
@@ -561,22 +568,28 @@ struct Batch {
         planDirty = false;
     }
 
-    template <typename Real>
-    void launchClass(int kdim, const BatchItem* items, int count, hipStream_t stream) {
+    template <typename Real, int KDIM>
+    void launchTiles(const BatchItem* items, int count, hipStream_t stream) {
         const int wpb = cfg.wavesPerBlock;
         const int want = (count + wpb - 1) / wpb;
         const int grid = std::max(1, std::min(want, cuCount * cfg.blocksPerCU));
-        const dim3 g(grid), b(wpb * kWave);
+        const size_t stageBytes = (size_t)wpb * batch_stage_elems<Real, KDIM>() * sizeof(Real);
+        hipLaunchKernelGGL((batch_tile_kernel<Real, KDIM>), dim3(grid), dim3(wpb * kWave), stageBytes, stream, dDesc,
+                           items, count, dTileSums);
+    }
+
+    template <typename Real>
+    void launchClass(int kdim, const BatchItem* items, int count, hipStream_t stream) {
         switch (kdim) {
-            case 0: hipLaunchKernelGGL((batch_tile_kernel<Real, 0>), g, b, 0, stream, dDesc, items, count, dTileSums); break;
-            case 1: hipLaunchKernelGGL((batch_tile_kernel<Real, 1>), g, b, 0, stream, dDesc, items, count, dTileSums); break;
-            case 2: hipLaunchKernelGGL((batch_tile_kernel<Real, 2>), g, b, 0, stream, dDesc, items, count, dTileSums); break;
-            case 3: hipLaunchKernelGGL((batch_tile_kernel<Real, 3>), g, b, 0, stream, dDesc, items, count, dTileSums); break;
-            case 4: hipLaunchKernelGGL((batch_tile_kernel<Real, 4>), g, b, 0, stream, dDesc, items, count, dTileSums); break;
-            case 5: hipLaunchKernelGGL((batch_tile_kernel<Real, 5>), g, b, 0, stream, dDesc, items, count, dTileSums); break;
-            case 6: hipLaunchKernelGGL((batch_tile_kernel<Real, 6>), g, b, 0, stream, dDesc, items, count, dTileSums); break;
-            case 8: hipLaunchKernelGGL((batch_tile_kernel<Real, 8>), g, b, 0, stream, dDesc, items, count, dTileSums); break;
-            case 10: hipLaunchKernelGGL((batch_tile_kernel<Real, 10>), g, b, 0, stream, dDesc, items, count, dTileSums); break;
+            case 0: launchTiles<Real, 0>(items, count, stream); break;
+            case 1: launchTiles<Real, 1>(items, count, stream); break;
+            case 2: launchTiles<Real, 2>(items, count, stream); break;
+            case 3: launchTiles<Real, 3>(items, count, stream); break;
+            case 4: launchTiles<Real, 4>(items, count, stream); break;
+            case 5: launchTiles<Real, 5>(items, count, stream); break;
+            case 6: launchTiles<Real, 6>(items, count, stream); break;
+            case 8: launchTiles<Real, 8>(items, count, stream); break;
+            case 10: launchTiles<Real, 10>(items, count, stream); break;
             default: throw Error(MDS_HIP_ERR_ARGUMENT, "unsupported kernel dimension");
         }
         check(hipGetLastError(), "batch tile kernel");

@@ -63,17 +63,22 @@ __device__ __forceinline__ void distance(double d2, double& d, double& invd) {
 }
 __device__ __forceinline__ void distance(float d2, float& d, float& invd) { distance32<HwSeeds>(d2, d, invd); }
 
-// Column locations of a tile.  When a column's P coordinates fit in 16
-// bytes (FP64 P <= 2, FP32 P <= 4) each wavefront stages its tile's 64
-// columns in LDS twice over, coordinate-major ([P][128]): step k of lane l
-// reads word l + k, so the step is an immediate offset and no lane index
-// wraps.  Larger P keep the columns in registers and rotate them one lane
-// per step (DPP), which costs no LDS and leaves occupancy alone.
+// Column locations of a tile.  When a column's P coordinates fit in
+// kStageBytes (FP32: P <= 10, FP64: P <= 2) each wavefront stages its tile's
+// 64 columns in LDS twice over, coordinate-major ([P][128]): step k of lane l
+// reads word l + k, so the step is an immediate offset, no lane index wraps
+// and no VALU instruction moves a column.  Larger P keep the columns in
+// registers and rotate them one lane per step (DPP, one VALU instruction per
+// 32-bit word), which costs no LDS.  The limits are measured: staging P = 5
+// and 10 in FP32 took 9 % off the tile kernel (N = 10,000: 75.6 -> 68.7 us at
+// P = 5, 119.2 -> 108.7 us at P = 10), while FP64 P = 5, whose table already
+// fills a quarter of the LDS, lost occupancy and ran 4 % slower.
 template <typename Real, int P>
 struct ColumnStage {
-    static constexpr bool kInLds = sizeof(Real) * P <= 16;
+    static constexpr int kStageBytes = sizeof(Real) == 4 ? 40 : 16;
+    static constexpr bool kInLds = sizeof(Real) * P <= kStageBytes;
     static constexpr int kElems = kInLds ? P * 2 * kTile : 1;  // per wavefront
-    static constexpr int kMaxElems = 16 / sizeof(Real) * 2 * kTile;  // any P that stages
+    static constexpr int kMaxElems = kStageBytes / sizeof(Real) * 2 * kTile;  // any P that stages
 };
 
 // Lane l writes column 64 J + l of x ([nb * 64][P]) into both halves.
