@@ -437,7 +437,7 @@ def _bench_leapfrog(steps, warmup, settle_ms, n=5000, p=2, tau=1.0):
         "tile_kernel_us": r["tile_ms"] * 1e3, "eval_us_events": r["eval_ms"] * 1e3, "tiles": r["tiles"]}))
 
 
-def _bench_surrogate(iterations=8000, n=20000, p=5, sweep_iterations=2000):
+def _bench_surrogate(iterations=8000, n=20000, p=5, sweep_iterations=2000, sweep=(2, 5, 10, 20), grid=True):
     """Surrogate-trajectory HMC against plain HMC: N = 20000, P = 5, FP64,
     euclid10 with 2% missing, truncation, precision fixed at its plug-in value
     at the posterior mode, trait precision I.  Warm-up: 100 plain iterations
@@ -454,7 +454,7 @@ def _bench_surrogate(iterations=8000, n=20000, p=5, sweep_iterations=2000):
 
     delta = datagen.euclid10(n, missing=0.02)
     r = compare_samplers(delta, p, iterations, warmup_evals=2000, leapfrog=20, truncation=True, epochs=4,
-                         sweep=(2, 5, 10, 20), sweep_iterations=sweep_iterations, grid=True,
+                         sweep=sweep, sweep_iterations=sweep_iterations, grid=grid,
                          log=lambda m: print(f"surrogate20000: {m}", file=sys.stderr, flush=True))
     del delta
     keys = ("accept_rate", "seconds", "ms_per_iteration", "min_ess", "min_ess_per_s", "surrogate_iterations",
@@ -482,7 +482,7 @@ def _bench_surrogate(iterations=8000, n=20000, p=5, sweep_iterations=2000):
         "best_tuned_ratio": r["best_tuned_ratio"], "best_tuned_z": r["best_tuned_z"],
         "columns": r["columns"],
     }
-    print(json.dumps(out))
+    return out
 
 
 def _bench_row_updates(moves=2000, n=10000, p=2):
@@ -735,7 +735,7 @@ def main():
         _bench_leapfrog(args.steps, args.warmup, args.settle_ms)
         return
     if args.workload == "surrogate20000":
-        _bench_surrogate(args.iterations, sweep_iterations=args.sweep_iterations)
+        print(json.dumps(_bench_surrogate(args.iterations, sweep_iterations=args.sweep_iterations)))
         return
     if args.workload == "rowupdate10000":
         _bench_row_updates()
@@ -818,6 +818,10 @@ def main():
             "batch64_uneven_fp32": _bench_batch(20, 3, no_cpu=args.no_cpu,
                                                 cpu_budget=min(args.cpu_budget, 10.0)),
             "hmc_n5000": _bench_hmc(),
+            # a short form of the surrogate workload (the full comparison, 8,000
+            # iterations per chain and the complete sweep, is --workload surrogate20000)
+            "surrogate_n20000_p5_short": _bench_surrogate(iterations=500, sweep_iterations=500, sweep=(2, 10),
+                                                          grid=False),
         }
     print(json.dumps(out))
 
