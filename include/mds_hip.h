@@ -27,7 +27,7 @@ extern "C" {
 #endif
 
 /* Flag bits, identical to the reference's mds::Flags (src/flags.h:6-14). */
-#define MDS_HIP_FLAG_FLOAT (1L << 2)           /* FP32 engine (compensated sums) */
+#define MDS_HIP_FLAG_FLOAT (1L << 2)           /* FP32 engine (FP32 terms, FP64 sums) */
 #define MDS_HIP_FLAG_TBB (1L << 3)             /* accepted, ignored by the GPU engine */
 #define MDS_HIP_FLAG_OPENCL (1L << 4)          /* "GPU engine": routed to HIP */
 #define MDS_HIP_FLAG_LEFT_TRUNCATION (1L << 5)

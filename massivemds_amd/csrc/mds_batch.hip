@@ -276,9 +276,10 @@ struct Batch {
     hipStream_t stream = nullptr;
     // One launch per dimension class runs its classes side by side: the
     // first on `stream`, the others on side streams forked from it and
-    // joined before the reduce, so one class's tail overlaps the next
-    // class's work (the results do not depend on the order: every tile
-    // writes its own slots).
+    // joined at the end, each class followed on its stream by the reduce of
+    // its own datasets, so one class's tail and reduce overlap the other
+    // classes' work (the results do not depend on the order: every tile
+    // writes its own slots, every dataset is reduced by one launch).
     static constexpr int kSideStreams = 3;
     hipStream_t side[kSideStreams] = {};
     hipEvent_t forkEvent = nullptr;
@@ -291,9 +292,10 @@ struct Batch {
     BatchItem* dItems = nullptr;
     ReduceItem* dReduce = nullptr;
     int numReduce = 0;
-    // launch plan: (kdim class or 0, first item, item count)
+    // launch plan: (kdim class or 0, first item, item count, first reduce
+    // item, reduce item count); each launch's reduce follows it on its stream
     struct Launch {
-        int kdim, first, count;
+        int kdim, first, count, redFirst, redCount;
     };
     std::vector<Launch> plan;
     BatchConfig cfg;
@@ -521,14 +523,18 @@ struct Batch {
             d.tileSumBase = b.tileSumBase;
             d.numTiles = b.numTiles;
         }
-        // items: per class (kdim, heaviest first), datasets interleaved run by run
+        // items: per class (kdim, heaviest first), datasets interleaved run by
+        // run; reduce items: the total and the row blocks of each dataset, in
+        // class order, so that a class's reduce is one contiguous range
         std::map<int, std::vector<int>, std::greater<int>> classes;
         for (size_t k = 0; k < e.size(); ++k) classes[e[k].kdim].push_back((int)k);
         std::vector<BatchItem> items;
+        std::vector<ReduceItem> red;
         plan.clear();
         const int tpi = std::max(1, cfg.tilesPerItem);
         for (auto& cls : classes) {
             const int first = (int)items.size();
+            const int redFirst = (int)red.size();
             std::vector<int> cursor(cls.second.size(), 0);
             bool more = true;
             while (more) {
@@ -546,14 +552,13 @@ struct Batch {
                     more = true;
                 }
             }
-            plan.push_back({cls.first, first, (int)items.size() - first});
+            for (int k : cls.second) {
+                red.push_back({k, -1});
+                for (int K = 0; K < e[k].nb; ++K) red.push_back({k, K});
+            }
+            plan.push_back({cls.first, first, (int)items.size() - first, redFirst, (int)red.size() - redFirst});
         }
-        if (cfg.strategy == 0) plan = {{0, 0, (int)items.size()}};
-        std::vector<ReduceItem> red;
-        for (size_t k = 0; k < e.size(); ++k) {
-            red.push_back({(int)k, -1});
-            for (int K = 0; K < e[k].nb; ++K) red.push_back({(int)k, K});
-        }
+        if (cfg.strategy == 0) plan = {{0, 0, (int)items.size(), 0, (int)red.size()}};
         numReduce = (int)red.size();
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         (void)hipFree(dItems);
@@ -615,18 +620,22 @@ struct Batch {
                 launchClass<float>(l.kdim, dItems + l.first, l.count, s);
             else
                 launchClass<double>(l.kdim, dItems + l.first, l.count, s);
+            // the class's datasets reduce while other classes still walk
+            if (l.redCount > 0) {
+                if (fp32)
+                    hipLaunchKernelGGL(batch_reduce_kernel<float>, dim3(l.redCount), dim3(kBatchReduceWaves * kWave), 0,
+                                       s, dDesc, dReduce + l.redFirst, dTileSums, dSums);
+                else
+                    hipLaunchKernelGGL(batch_reduce_kernel<double>, dim3(l.redCount),
+                                       dim3(kBatchReduceWaves * kWave), 0, s, dDesc, dReduce + l.redFirst, dTileSums,
+                                       dSums);
+                check(hipGetLastError(), "batch reduce kernel");
+            }
         }
         for (int k = 0; k < used; ++k) {
             check(hipEventRecord(joinEvent[k], side[k]), "hipEventRecord(join)");
             check(hipStreamWaitEvent(stream, joinEvent[k], 0), "hipStreamWaitEvent(join)");
         }
-        if (fp32)
-            hipLaunchKernelGGL(batch_reduce_kernel<float>, dim3(numReduce), dim3(kBatchReduceWaves * kWave), 0, stream,
-                               dDesc, dReduce, dTileSums, dSums);
-        else
-            hipLaunchKernelGGL(batch_reduce_kernel<double>, dim3(numReduce), dim3(kBatchReduceWaves * kWave), 0,
-                               stream, dDesc, dReduce, dTileSums, dSums);
-        check(hipGetLastError(), "batch reduce kernel");
     }
 
     void evaluate() {

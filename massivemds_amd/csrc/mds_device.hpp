@@ -1,5 +1,5 @@
 // Device-side building blocks for the gfx950 MDS kernels: wavefront
-// reductions and rotations on DPP (no LDS, no atomics) and compensated
+// reductions and rotations on DPP (no LDS, no atomics) and blocked
 // accumulators.
 #pragma once
 
@@ -78,8 +78,8 @@ __device__ __forceinline__ double read_last_lane(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// Per-lane accumulators: plain FP64, or Kahan-compensated FP32 whose value is
-// promoted to FP64 when it leaves the lane.
+// Per-lane accumulators: plain FP64, or FP32 group sums flushed into an FP64
+// sum whose value leaves the lane in FP64.
 template <typename Real>
 struct LaneSum;
 
@@ -88,10 +88,11 @@ struct LaneSum;
 // fuses a product into the sum).  flush() closes a group of steps; value()
 // is read after a flush.
 //
-// FP64: a plain sum.  FP32: blocked compensated summation: within a group the
-// terms go into a plain FP32 group sum g, and each flush adds g into a
-// Kahan-compensated (s, c) pair, so the compensation costs four operations
-// per group instead of per term.
+// FP64: a plain sum.  FP32: blocked summation: within a group the terms go
+// into a plain FP32 group sum g, and each flush adds g, widened exactly, into
+// an FP64 sum s (two instructions per group; s takes the two registers a
+// compensated FP32 pair would, and its rounding error is far below the group
+// sums').
 template <>
 struct LaneSum<double> {
     double s = 0.0;
@@ -101,26 +102,17 @@ struct LaneSum<double> {
     __device__ __forceinline__ double value() const { return s; }
 };
 
-// (s, c) += g with the Kahan step (c holds the negated rounding error)
-__device__ __forceinline__ void kahan_add(float& s, float& c, float g) {
-    const float y = g - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
-
 template <>
 struct LaneSum<float> {
-    float s = 0.0f;
-    float c = 0.0f;
+    double s = 0.0;
     float g = 0.0f;
     __device__ __forceinline__ void add(float v) { g += v; }
     __device__ __forceinline__ void add_product(float a, float b) { g = fmaf(a, b, g); }
     __device__ __forceinline__ void flush() {
-        kahan_add(s, c, g);
+        s += (double)g;
         g = 0.0f;
     }
-    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+    __device__ __forceinline__ double value() const { return s; }
 };
 
 __device__ __forceinline__ bool m_isnan(double v) { return v != v; }
@@ -130,9 +122,9 @@ __device__ __forceinline__ bool m_isnan(float v) { return v != v; }
 // the partial sum of tile column (l + k) mod 64; rotate() moves every
 // accumulator one lane down (DPP wave_rol:1, lane l <- lane l + 1) so that
 // at step k + 1 lane l holds column (l + k + 1) mod 64.  After 64 steps lane
-// l holds column l.  FP32 rotates only the group sum g each step; the
-// compensated pair stays put during a group of STEPS steps and moves STEPS
-// lanes at the flush (one ds_bpermute each), landing on g's columns.
+// l holds column l.  FP32 rotates only the group sum g each step; the FP64
+// sum stays put during a group of STEPS steps and moves STEPS lanes at the
+// flush (one ds_bpermute per 32-bit half), landing on g's columns.
 constexpr int kDppWaveRol1 = 0x134;
 
 template <typename Real>
@@ -166,20 +158,17 @@ __device__ __forceinline__ double lane_rotate(double v) {
 
 template <>
 struct ColSum<float> {
-    float s = 0.0f;
-    float c = 0.0f;
+    double s = 0.0;
     float g = 0.0f;
     __device__ __forceinline__ void add(float v) { g += v; }
     __device__ __forceinline__ void add_product(float a, float b) { g = fmaf(a, b, g); }
     __device__ __forceinline__ void rotate() { g = dpp_rotate<kDppWaveRol1>(g); }
     template <int STEPS>
     __device__ __forceinline__ void flush() {
-        s = lane_rotate<STEPS>(s);
-        c = lane_rotate<STEPS>(c);
-        kahan_add(s, c, g);
+        s = lane_rotate<STEPS>(s) + (double)g;
         g = 0.0f;
     }
-    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+    __device__ __forceinline__ double value() const { return s; }
 };
 
 }  // namespace mdship

@@ -65,33 +65,45 @@ __device__ __forceinline__ void distance(float d2, float& d, float& invd) { dist
 
 // Column locations of a tile.  When a column's P coordinates fit in
 // kStageBytes (FP32: P <= 10, FP64: P <= 2) each wavefront stages its tile's
-// 64 columns in LDS twice over, coordinate-major ([P][128]): step k of lane l
-// reads word l + k, so the step is an immediate offset, no lane index wraps
-// and no VALU instruction moves a column.  Larger P keep the columns in
-// registers and rotate them one lane per step (DPP, one VALU instruction per
-// 32-bit word), which costs no LDS.  The limits are measured: staging P = 5
-// and 10 in FP32 took 9 % off the tile kernel (N = 10,000: 75.6 -> 68.7 us at
-// P = 5, 119.2 -> 108.7 us at P = 10), while FP64 P = 5, whose table already
-// fills a quarter of the LDS, lost occupancy and ran 4 % slower.
+// 64 columns in LDS twice over, so that step k of lane l reads column l + k
+// with no lane index wrapping and no VALU instruction moving a column.  For
+// P <= 2 the stage is coordinate-major ([P][128]: coordinate p of column c at
+// p * 128 + c, both coordinates of a step within the 8-bit offsets of one
+// ds_read2); for larger P it is column-major ([128][P]: at c * P + p), so
+// that every read of a group of steps is an immediate offset from one address
+// per group (coordinate-major rows 512 bytes apart would need an address add
+// per coordinate and step).  In column-major reads the lanes are P words
+// apart: conflict-free for P = 5 (an odd number of words) and P = 10 (of
+// doublewords); even P may see two-way bank conflicts.
+// Larger P keep the columns in registers and rotate them one lane per step
+// (DPP, one VALU instruction per 32-bit word), which costs no LDS.  The limits
+// are measured: staging P = 5 and 10 in FP32 took 9 % off the tile kernel
+// (N = 10,000: 75.6 -> 68.7 us at P = 5, 119.2 -> 108.7 us at P = 10), while
+// FP64 P = 5, whose table already fills a quarter of the LDS, lost occupancy
+// and ran 4 % slower.
 template <typename Real, int P>
 struct ColumnStage {
     static constexpr int kStageBytes = sizeof(Real) == 4 ? 40 : 16;
     static constexpr bool kInLds = sizeof(Real) * P <= kStageBytes;
+    static constexpr bool kColumnMajor = kInLds && P > 2;
     static constexpr int kElems = kInLds ? P * 2 * kTile : 1;  // per wavefront
     static constexpr int kMaxElems = kStageBytes / sizeof(Real) * 2 * kTile;  // any P that stages
+    // offset of coordinate p of stage column c
+    __device__ static constexpr int at(int c, int p) { return kColumnMajor ? c * P + p : p * 2 * kTile + c; }
 };
 
 // Lane l writes column 64 J + l of x ([nb * 64][P]) into both halves.
 template <typename Real, int P>
 __device__ __forceinline__ void stage_columns(Real* __restrict__ stage, const Real* __restrict__ x, int J) {
-    if constexpr (ColumnStage<Real, P>::kInLds) {
+    using CS = ColumnStage<Real, P>;
+    if constexpr (CS::kInLds) {
         const int lane = threadIdx.x % kWave;
         const Real* src = x + ((int64_t)J * kTile + lane) * P;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const Real v = src[p];
-            stage[p * 2 * kTile + lane] = v;
-            stage[p * 2 * kTile + kTile + lane] = v;
+            stage[CS::at(lane, p)] = v;
+            stage[CS::at(lane + kTile, p)] = v;
         }
     }
 }
@@ -202,7 +214,8 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
     constexpr int kFirst = DIAG ? 1 : 0;                 // first step
     constexpr int kSteps = DIAG ? kTile / 2 : kTile;     // steps walked
     constexpr int kLast = kFirst + kSteps - 1;
-    constexpr int kFlushSteps = 4 * G;                   // steps per compensated update (FP32)
+    constexpr int kFlushSteps = 8 * G;                   // steps per FP32 group sum
+    constexpr int kFlushPasses = kFlushSteps / (2 * G);  // loop passes per FP32 group sum
     static_assert(kSteps % kFlushSteps == 0, "the walk must end on a flush");
     const int lane = threadIdx.x % kWave;
     const Real* tileStep = tileBase + lane;
@@ -228,16 +241,18 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
         xi[p] = xrow[p];
         if constexpr (!kStaged) xj[p] = xcol[p];
     }
-    const Real* stageLane = stage + lane;
+    using CS = ColumnStage<Real, P>;
+    const Real* stageLane = stage + CS::at(lane, 0);
 
     TileAcc<Real, P> acc;
     auto run_group = [&](const Real (&b)[G], const Real (&bT)[G], int k0) {
         Real xs[G][P];
         if constexpr (kStaged) {
+            const Real* stageGroup = stageLane + CS::at(k0, 0);
 #pragma unroll
             for (int g = 0; g < G; ++g)
 #pragma unroll
-                for (int p = 0; p < P; ++p) xs[g][p] = stageLane[p * 2 * kTile + k0 + g];
+                for (int p = 0; p < P; ++p) xs[g][p] = stageGroup[CS::at(g, p)];
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -262,10 +277,10 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
         load_group(bufA, bufAT, k0 + 2 * G);
         __builtin_amdgcn_sched_barrier(0);
         run_group(bufB, bufBT, k0 + G);
-        // every second pass (4 G steps) the FP32 group sums enter their
-        // compensated sums (FP64: nothing to do); the walk's step count is a
-        // multiple of 4 G, so its last pass flushes
-        if (((k0 - kFirst) / (2 * G)) % 2 == 1) {
+        // every fourth pass (8 G steps) the FP32 group sums enter their FP64
+        // sums (FP64: nothing to do); the walk's step count is a multiple of
+        // 8 G, so its last pass flushes
+        if (((k0 - kFirst) / (2 * G)) % kFlushPasses == kFlushPasses - 1) {
             acc.s.flush();
 #pragma unroll
             for (int p = 0; p < P; ++p) {
