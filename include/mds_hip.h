@@ -311,8 +311,9 @@ int mds_hip_hmc_run(mds_hip_handle h, const mds_hip_hmc_options* options, const 
  * The sampler above on every entry of a batch at once (no counterpart in the
  * reference; many chains, of one dataset through
  * mds_hip_batch_share_pairwise_data or of datasets of uneven N and P): one
- * batched tile launch, one batched reduce and one kick launch per leapfrog
- * step move every chain, the trajectory captured as one hipGraph.  Entry k
+ * batched tile launch and one batched reduce (which also applies the kicks)
+ * per leapfrog step move every chain, the trajectory captured as one
+ * hipGraph.  Entry k
  * runs its own chain with seed options->seed + k, its own precision (from
  * its setParameters), step size and trait precision, and gives the bits of
  * mds_hip_hmc_run on a single engine with that seed and the same data.

@@ -49,6 +49,8 @@ struct BatchDataset {
     int64_t planeStride;
     double tau;
     const double* devTau;   // if set, the precision is read here (a sampler's parameter block)
+    double* kickMom;        // a sampler's momenta [n][p] and parameter block, for reduces that
+    const double* kickParams;  // kick each gradient row they write (null otherwise)
     int nb, n, p, kdim;
     int truncated, symmetric;
     int tileSumBase, numTiles;
@@ -155,8 +157,11 @@ constexpr int kBatchReduceWaves = 8;
 // kAhead column blocks of all KD coordinates in flight together), then the
 // wavefront partials are combined per coordinate in wavefront order.  Per
 // coordinate the additions are those of the single-instance reduce, in the
-// same order, so the rows are bitwise the same.
-template <typename PartT, int KD>
+// same order, so the rows are bitwise the same.  KICK >= 0: each row's
+// thread then applies the sampler's kick of that form to its row, as the
+// chains' kick kernel would (kick_row: the same bits, one launch fewer per
+// leapfrog step; no reduce block reads locations).
+template <typename PartT, int KD, int KICK>
 __device__ __forceinline__ void batch_reduce_rows(const BatchDataset& d, int K, double* red) {
     constexpr int kAhead = 4;
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
@@ -196,9 +201,12 @@ __device__ __forceinline__ void batch_reduce_rows(const BatchDataset& d, int K, 
         }
         __syncthreads();
     }
+    if constexpr (KICK >= 0) {
+        if (wave == 0) kick_row<PartT, KICK>((PartT*)d.x, d.kickMom, d.grad, d.kickParams, d.n, d.p, KD, row);
+    }
 }
 
-template <typename PartT>
+template <typename PartT, int KICK>
 __global__ __launch_bounds__(kBatchReduceWaves * kWave) void batch_reduce_kernel(
     const BatchDataset* __restrict__ ds, const ReduceItem* __restrict__ items,
     const double* __restrict__ tileSums, double* __restrict__ sums) {
@@ -231,14 +239,14 @@ __global__ __launch_bounds__(kBatchReduceWaves * kWave) void batch_reduce_kernel
         return;
     }
     switch (d.kdim) {  // uniform over the block
-        case 1: batch_reduce_rows<PartT, 1>(d, item.K, red); break;
-        case 2: batch_reduce_rows<PartT, 2>(d, item.K, red); break;
-        case 3: batch_reduce_rows<PartT, 3>(d, item.K, red); break;
-        case 4: batch_reduce_rows<PartT, 4>(d, item.K, red); break;
-        case 5: batch_reduce_rows<PartT, 5>(d, item.K, red); break;
-        case 6: batch_reduce_rows<PartT, 6>(d, item.K, red); break;
-        case 8: batch_reduce_rows<PartT, 8>(d, item.K, red); break;
-        case 10: batch_reduce_rows<PartT, 10>(d, item.K, red); break;
+        case 1: batch_reduce_rows<PartT, 1, KICK>(d, item.K, red); break;
+        case 2: batch_reduce_rows<PartT, 2, KICK>(d, item.K, red); break;
+        case 3: batch_reduce_rows<PartT, 3, KICK>(d, item.K, red); break;
+        case 4: batch_reduce_rows<PartT, 4, KICK>(d, item.K, red); break;
+        case 5: batch_reduce_rows<PartT, 5, KICK>(d, item.K, red); break;
+        case 6: batch_reduce_rows<PartT, 6, KICK>(d, item.K, red); break;
+        case 8: batch_reduce_rows<PartT, 8, KICK>(d, item.K, red); break;
+        case 10: batch_reduce_rows<PartT, 10, KICK>(d, item.K, red); break;
         default: break;  // the host only admits kernel dimensions <= 10 here
     }
 }
@@ -257,6 +265,8 @@ struct BatchEntry {
     bool symmetric = true, dataSet = false, precisionSet = false;
     double tau = 0.0;
     const double* devTau = nullptr;  // set by a sampler for the length of its run
+    double* kickMom = nullptr;       // likewise: the chain's momenta and parameter
+    const double* kickParams = nullptr;  // block, for reduces that kick (enqueue)
     void* dDelta = nullptr;          // owned packed observations (null while shared)
     size_t deltaBytes = 0;
     int shareSource = -1;            // >= 0: reads that entry's packed observations
@@ -514,6 +524,8 @@ struct Batch {
             d.planeStride = (int64_t)b.numTiles * kTileElems;
             d.tau = b.tau;
             d.devTau = b.devTau;
+            d.kickMom = b.kickMom;
+            d.kickParams = b.kickParams;
             d.nb = b.nb;
             d.n = b.N;
             d.p = b.P;
@@ -600,7 +612,30 @@ struct Batch {
         check(hipGetLastError(), "batch tile kernel");
     }
 
-    void enqueue() {
+    template <int KICK>
+    void launchReduceKick(const ReduceItem* items, int count, hipStream_t s) {
+        if (fp32)
+            hipLaunchKernelGGL((batch_reduce_kernel<float, KICK>), dim3(count), dim3(kBatchReduceWaves * kWave), 0,
+                               s, dDesc, items, dTileSums, dSums);
+        else
+            hipLaunchKernelGGL((batch_reduce_kernel<double, KICK>), dim3(count), dim3(kBatchReduceWaves * kWave), 0,
+                               s, dDesc, items, dTileSums, dSums);
+    }
+
+    void launchReduce(const ReduceItem* items, int count, hipStream_t s, int kickForm) {
+        switch (kickForm) {
+            case -1: launchReduceKick<-1>(items, count, s); break;
+            case kKick: launchReduceKick<kKick>(items, count, s); break;
+            case kKick2Drift: launchReduceKick<kKick2Drift>(items, count, s); break;
+            default: throw Error(MDS_HIP_ERR_ARGUMENT, "unsupported kick form");
+        }
+        check(hipGetLastError(), "batch reduce kernel");
+    }
+
+    // Every entry's likelihood and gradient; kickForm >= 0 (a sampler's
+    // leapfrog step, with every entry's kickMom and kickParams set): the
+    // reduces also apply that kick to every gradient row.
+    void enqueue(int kickForm = -1) {
         if (planDirty) buildPlan();
         int used = 0;  // side streams forked
         if (plan.size() > 1) check(hipEventRecord(forkEvent, stream), "hipEventRecord(fork)");
@@ -621,16 +656,7 @@ struct Batch {
             else
                 launchClass<double>(l.kdim, dItems + l.first, l.count, s);
             // the class's datasets reduce while other classes still walk
-            if (l.redCount > 0) {
-                if (fp32)
-                    hipLaunchKernelGGL(batch_reduce_kernel<float>, dim3(l.redCount), dim3(kBatchReduceWaves * kWave), 0,
-                                       s, dDesc, dReduce + l.redFirst, dTileSums, dSums);
-                else
-                    hipLaunchKernelGGL(batch_reduce_kernel<double>, dim3(l.redCount),
-                                       dim3(kBatchReduceWaves * kWave), 0, s, dDesc, dReduce + l.redFirst, dTileSums,
-                                       dSums);
-                check(hipGetLastError(), "batch reduce kernel");
-            }
+            if (l.redCount > 0) launchReduce(dReduce + l.redFirst, l.redCount, s, kickForm);
         }
         for (int k = 0; k < used; ++k) {
             check(hipEventRecord(joinEvent[k], side[k]), "hipEventRecord(join)");
@@ -919,7 +945,11 @@ struct BatchHmc {
             // is captured on the batch's stream), the precision read from each
             // chain's parameter block
             b.cfg.strategy = 0;
-            for (int c = 0; c < C; ++c) b.e[c].devTau = dParams + (size_t)c * kHmcParamCount + kHmcTau;
+            for (int c = 0; c < C; ++c) {
+                b.e[c].devTau = dParams + (size_t)c * kHmcParamCount + kHmcTau;
+                b.e[c].kickMom = dMom + ch[c].momOffset;
+                b.e[c].kickParams = dParams + (size_t)c * kHmcParamCount;
+            }
             b.planDirty = true;
             b.buildPlan();
         } catch (...) {
@@ -957,7 +987,11 @@ struct BatchHmc {
         hParams = hEnergy0 = hEnergy = hSums = hRows = hGram = nullptr;
         uploadDone = nullptr;
         // the batch evaluates with the host precisions again
-        for (BatchEntry& en : b.e) en.devTau = nullptr;
+        for (BatchEntry& en : b.e) {
+            en.devTau = nullptr;
+            en.kickMom = nullptr;
+            en.kickParams = nullptr;
+        }
         b.cfg = savedCfg;
         b.planDirty = true;
     }
@@ -999,15 +1033,11 @@ struct BatchHmc {
     // Every chain's likelihood and gradient at its current locations.
     void evaluateAll() { b.enqueue(); }
 
+    // The reduces kick every chain's rows right after writing them (the
+    // kick_row arithmetic of the kick kernel: the same bits).
     void trajectory() {
         kick<kKickDrift>();
-        for (int s = 0; s < opt.leapfrogSteps; ++s) {
-            evaluateAll();
-            if (s + 1 < opt.leapfrogSteps)
-                kick<kKick2Drift>();
-            else
-                kick<kKick>();
-        }
+        for (int s = 0; s < opt.leapfrogSteps; ++s) b.enqueue(s + 1 < opt.leapfrogSteps ? kKick2Drift : kKick);
     }
 
     // The whole trajectory of every chain as one graph; every exit ends the capture.

@@ -38,6 +38,14 @@
 
 namespace mdship {
 
+// An HMC kick applied by the reduce to each gradient row it writes (see
+// ReduceLaunch): form -1 for none.
+struct ReduceKick {
+    int form = -1;
+    double* mom = nullptr;
+    const double* params = nullptr;
+};
+
 struct Engine {
     int P = 0, N = 0, kdim = 0, device = 0;
     long flags = 0;
@@ -465,9 +473,11 @@ struct Engine {
     }
 
     // The reduce kernel: the gradient rows into grad (when gradient) and
-    // (0.5 * the sum of the tile sums) * sumScale into *sumOut.
+    // (0.5 * the sum of the tile sums) * sumScale into *sumOut; with a kick,
+    // each row then moves its momentum (and location) at the current
+    // locations.
     void launch_reduce_pass(bool gradient, const double* tileSums, const void* part, double* grad, double* sumOut,
-                            double sumScale) {
+                            double sumScale, const ReduceKick& kick = ReduceKick()) {
         ReduceLaunch r;
         r.part = part;
         r.fp32 = fp32;
@@ -481,10 +491,14 @@ struct Engine {
         r.grad = grad;
         r.sumOut = sumOut;
         r.sumScale = sumScale;
+        r.kickForm = kick.form;
+        r.kickX = dX[cur];
+        r.kickMom = kick.mom;
+        r.kickParams = kick.params;
         check(launch_reduce(r, stream), "reduce kernel");
     }
 
-    void evaluate(bool gradient, const double* devTau = nullptr) {
+    void evaluate(bool gradient, const double* devTau = nullptr, const ReduceKick& kick = ReduceKick()) {
         if (!dataSet) throw Error(MDS_HIP_ERR_STATE, "pairwise data not set");
         if (!precisionSet) throw Error(MDS_HIP_ERR_STATE, "precision not set");
         flush_row_write();
@@ -497,7 +511,7 @@ struct Engine {
         }
         launch_tile_pass(gradient, devTau, dTileSums, dPart);
         if (timing) check(hipEventRecord(e1, stream), "hipEventRecord");
-        launch_reduce_pass(gradient, dTileSums, dPart, dGrad, dSum, 1.0);
+        launch_reduce_pass(gradient, dTileSums, dPart, dGrad, dSum, 1.0, kick);
         if (timing) {
             check(hipEventRecord(e2, stream), "hipEventRecord");
             pending.push_back(e0);

@@ -266,12 +266,25 @@ void Hmc::kick() {
 }
 
 // One trajectory of opt.leapfrogSteps steps from the gradient now in
-// e.dGrad: kick + drift, then per step the gradient and the kicks.
+// e.dGrad: kick + drift, then per step the gradient and the kicks.  With
+// exact gradients the reduce applies each step's kicks to the rows it has
+// just written (the same kick_row arithmetic as the kick kernel, so the same
+// bits), one launch fewer per step; collecting snapshots needs the gradient
+// at the locations before the drift, and the surrogate has no reduce.
 void Hmc::trajectory(int mode) {
     kick<kKickDrift>();
     for (int s = 0; s < opt.leapfrogSteps; ++s) {
+        const int form = s + 1 < opt.leapfrogSteps ? kKick2Drift : kKick;
+        if (mode == kExact) {
+            ReduceKick k;
+            k.form = form;
+            k.mom = dMom;
+            k.params = dParams;
+            e.evaluate(true, dParams + kHmcTau, k);
+            continue;
+        }
         gradient(mode);
-        if (s + 1 < opt.leapfrogSteps)
+        if (form == kKick2Drift)
             kick<kKick2Drift>();
         else
             kick<kKick>();

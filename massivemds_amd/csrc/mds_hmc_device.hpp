@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mds_device.hpp"  // floating-point contraction within single expressions, before any code here
 #include "mds_hmc.hpp"
 
 namespace mdship {

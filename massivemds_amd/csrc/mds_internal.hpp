@@ -55,6 +55,14 @@ struct ReduceLaunch {
     double* grad;            // [n][p]
     double* sumOut;          // 1 double: (0.5 * sum of tile sums) * sumScale
     double sumScale = 1.0;
+    // Optional HMC kick of each gradient row right after it is written
+    // (kickForm: a KickForm of the sampler, or -1 for none): the locations
+    // (the engine's real type, [n][kdim]), the momenta [n][p] and the
+    // sampler's parameter block.
+    int kickForm = -1;
+    void* kickX = nullptr;
+    double* kickMom = nullptr;
+    const double* kickParams = nullptr;
 };
 
 // Change of the raw sum of increments when one location row moves (O(N)):

@@ -18,6 +18,7 @@
 // Each tile writes its likelihood partial and its two 64-row gradient
 // partials to fixed slots; a second kernel adds them in a fixed order in
 // FP64.  No atomics anywhere, so results are bitwise reproducible.
+#include "mds_hmc_device.hpp"
 #include "mds_internal.hpp"
 #include "mds_tile.hpp"
 
@@ -55,10 +56,15 @@ constexpr int kReduceThreads = kReduceWaves * kWave;
 // Last block: the likelihood, a fixed-order FP64 sum of the tile partials,
 // halved as in src/NewMultiDimensionalScaling.hpp:556, then scaled by
 // sumScale (1, or 0.5 tau for the untruncated value of :198-202).
-template <typename PartT, int P>
+// KICK >= 0: each row's thread then applies the sampler's kick of that form
+// to its row (kick_row, reading back the gradient it has just written), so
+// that a leapfrog step needs no kick launch of its own; the reduce blocks
+// read no locations, so moving them here races with nothing.
+template <typename PartT, int P, int KICK>
 __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
     const PartT* __restrict__ part, const double* __restrict__ tileSums, int numTiles,
-    int nb, int n, int pOut, double* __restrict__ grad, double* __restrict__ sumOut, double sumScale) {
+    int nb, int n, int pOut, double* __restrict__ grad, double* __restrict__ sumOut, double sumScale,
+    PartT* __restrict__ kickX, double* __restrict__ kickMom, const double* __restrict__ kickParams) {
     __shared__ double red[kReduceThreads * (P > 1 ? P : 1)];
     const int tid = threadIdx.x;
     // Loads are issued kAhead iterations at a time and added in the same
@@ -122,6 +128,7 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
             for (int w = 0; w < kReduceWaves; ++w) v += red[(p * kReduceWaves + w) * kWave + lane];
             if (row < n && p < pOut) grad[(int64_t)row * pOut + p] = v;
         }
+        if constexpr (KICK >= 0) kick_row<PartT, KICK>(kickX, kickMom, grad, kickParams, n, pOut, P, row);
     }
 }
 
@@ -166,16 +173,29 @@ static hipError_t dispatch_dim(const TileLaunch& a, hipStream_t s) {
 hipError_t launch_tiles_f64(const TileLaunch& a, hipStream_t s) { return dispatch_dim<double>(a, s); }
 hipError_t launch_tiles_f32(const TileLaunch& a, hipStream_t s) { return dispatch_dim<float>(a, s); }
 
+template <typename PartT, int P, int KICK>
+static void reduce_go(const ReduceLaunch& a, hipStream_t s) {
+    const dim3 grid(a.gradient ? a.nb + 1 : 1);
+    hipLaunchKernelGGL((reduce_kernel<PartT, P, KICK>), grid, dim3(kReduceThreads), 0, s, (const PartT*)a.part,
+                       a.tileSums, a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut, a.sumScale, (PartT*)a.kickX,
+                       a.kickMom, a.kickParams);
+}
+
+template <typename PartT, int P>
+static hipError_t reduce_kick(const ReduceLaunch& a, hipStream_t s) {
+    switch (a.kickForm) {
+        case -1: reduce_go<PartT, P, -1>(a, s); break;
+        case kKick: reduce_go<PartT, P, kKick>(a, s); break;
+        case kKick2Drift: reduce_go<PartT, P, kKick2Drift>(a, s); break;
+        default: return hipErrorInvalidValue;  // the kick that starts a trajectory has no gradient before it
+    }
+    return hipGetLastError();
+}
+
 template <int P>
 static hipError_t reduce_p(const ReduceLaunch& a, hipStream_t s) {
-    const dim3 grid(a.gradient ? a.nb + 1 : 1);
-    if (a.fp32)
-        hipLaunchKernelGGL((reduce_kernel<float, P>), grid, dim3(kReduceThreads), 0, s, (const float*)a.part,
-                           a.tileSums, a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut, a.sumScale);
-    else
-        hipLaunchKernelGGL((reduce_kernel<double, P>), grid, dim3(kReduceThreads), 0, s, (const double*)a.part,
-                           a.tileSums, a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut, a.sumScale);
-    return hipGetLastError();
+    if (a.kickForm >= 0 && (!a.gradient || !a.kickX || !a.kickMom || !a.kickParams)) return hipErrorInvalidValue;
+    return a.fp32 ? reduce_kick<float, P>(a, s) : reduce_kick<double, P>(a, s);
 }
 
 hipError_t launch_reduce(const ReduceLaunch& a, hipStream_t s) {
