@@ -60,14 +60,22 @@ struct Engine {
     bool dataSet = false;
     void* dX[2] = {nullptr, nullptr};
     int cur = 0;
+    // storeState keeps the stored locations in dX[1 - cur], lazily: it only
+    // sets storePending (the stored state is still dX[cur] itself); a write
+    // of every location then goes to the other buffer and flips cur (no
+    // copy), and a write of part of dX[cur] copies it to dX[1 - cur] first.
+    bool storePending = false;
     uint32_t* dTiles = nullptr;
     double* dTileSums = nullptr;
     void* dPart = nullptr;     // gradient partials in the engine's real type
     double* dGrad = nullptr;
-    double* dSum = nullptr;
+    double* dSum = nullptr;    // dGrad + N * P: the sum follows the gradient, so one copy reads both
     double* hGrad = nullptr;   // pinned
     double* hSum = nullptr;    // pinned
-    void* hX = nullptr;        // pinned staging for location uploads
+    void* hXBuf[2] = {nullptr, nullptr};  // pinned staging for location uploads, used in turn
+    hipEvent_t hXDone[2] = {nullptr, nullptr};  // the last upload from each has run
+    bool hXBusy[2] = {false, false};
+    int hXNext = 0;
 
     bool precisionSet = false;
     double precision = 0.0, storedPrecision = 0.0;
@@ -177,21 +185,25 @@ struct Engine {
                   "hipMemcpy(tiles)");
             check(hipMalloc((void**)&dTileSums, tiles.size() * sizeof(double)), "hipMalloc(sums)");
             check(hipMalloc(&dPart, (size_t)nb * nb * kdim * kTile * realSize), "hipMalloc(partials)");
-            check(hipMalloc((void**)&dGrad, (size_t)N * P * sizeof(double)), "hipMalloc(gradient)");
-            check(hipMalloc((void**)&dSum, sizeof(double)), "hipMalloc(sum)");
-            check(hipHostMalloc((void**)&hGrad, (size_t)N * P * sizeof(double), hipHostMallocDefault),
+            check(hipMalloc((void**)&dGrad, ((size_t)N * P + 1) * sizeof(double)), "hipMalloc(gradient)");
+            dSum = dGrad + (size_t)N * P;
+            // the host copies hold the sum after the gradient as well
+            check(hipHostMalloc((void**)&hGrad, ((size_t)N * P + 1) * sizeof(double), hipHostMallocDefault),
                   "hipHostMalloc(gradient)");
-            check(hipHostMalloc((void**)&hGradStored, (size_t)N * P * sizeof(double), hipHostMallocDefault),
+            check(hipHostMalloc((void**)&hGradStored, ((size_t)N * P + 1) * sizeof(double), hipHostMallocDefault),
                   "hipHostMalloc(stored gradient)");
             check(hipHostMalloc((void**)&hSum, sizeof(double), hipHostMallocDefault), "hipHostMalloc(sum)");
-            check(hipHostMalloc(&hX, (size_t)npad * kdim * realSize, hipHostMallocDefault),
-                  "hipHostMalloc(locations)");
+            for (int k = 0; k < 2; ++k) {
+                check(hipHostMalloc(&hXBuf[k], (size_t)npad * kdim * realSize, hipHostMallocDefault),
+                      "hipHostMalloc(locations)");
+                check(hipEventCreateWithFlags(&hXDone[k], hipEventDisableTiming), "hipEventCreate(locations)");
+                std::memset(hXBuf[k], 0, (size_t)npad * kdim * realSize);
+            }
             // written by the row kernel itself (coherent pinned memory, no copy)
             check(hipHostMalloc((void**)&hRowPartial, row_delta_blocks(N) * sizeof(double), hipHostMallocCoherent),
                   "hipHostMalloc(row sums)");
             check(hipMalloc((void**)&dRowDone, sizeof(unsigned int)), "hipMalloc(row counter)");
             check(hipMemsetAsync(dRowDone, 0, sizeof(unsigned int), stream), "hipMemset(row counter)");
-            std::memset(hX, 0, (size_t)npad * kdim * realSize);
             check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         } catch (...) {
             release();
@@ -214,12 +226,17 @@ struct Engine {
         (void)hipFree(dTiles);
         (void)hipFree(dTileSums);
         (void)hipFree(dPart);
-        (void)hipFree(dGrad);
-        (void)hipFree(dSum);
+        (void)hipFree(dGrad);  // dSum lies within
         (void)hipHostFree(hGrad);
         (void)hipHostFree(hGradStored);
         (void)hipHostFree(hSum);
-        (void)hipHostFree(hX);
+        for (int k = 0; k < 2; ++k) {
+            (void)hipHostFree(hXBuf[k]);
+            if (hXDone[k]) (void)hipEventDestroy(hXDone[k]);
+            hXBuf[k] = nullptr;
+            hXDone[k] = nullptr;
+            hXBusy[k] = false;
+        }
         (void)hipHostFree(hRowPartial);
         (void)hipFree(dRowDone);
         dRowDone = nullptr;
@@ -229,7 +246,6 @@ struct Engine {
         dTiles = nullptr;
         dTileSums = dGrad = dSum = hGrad = hGradStored = hSum = nullptr;
         dPart = nullptr;
-        hX = nullptr;
         stream = nullptr;
     }
 
@@ -273,31 +289,57 @@ struct Engine {
             rowWritePending = true;
             rowWriteIndex = index;
         } else {
-            // Stage as [N][kdim] in the engine's real type, padding with zeros.
+            // Stage as [N][kdim] in the engine's real type, padding with zeros,
+            // in the staging buffer whose last upload has run (no wait for
+            // this one: the next call stages in the other buffer).
             rowWritePending = false;  // every row is overwritten
             pendingRow = -1;
+            const int k = hXNext;
+            if (hXBusy[k]) check(hipEventSynchronize(hXDone[k]), "hipEventSynchronize(locations)");
+            hXBusy[k] = false;
             if (fp32) {
-                float* st = (float*)hX;
+                float* st = (float*)hXBuf[k];
                 for (size_t r = 0; r < rows; ++r)
-                    for (int k = 0; k < kdim; ++k) st[r * kdim + k] = k < P ? (float)loc[r * P + k] : 0.0f;
+                    for (int c = 0; c < kdim; ++c) st[r * kdim + c] = c < P ? (float)loc[r * P + c] : 0.0f;
             } else {
-                double* st = (double*)hX;
+                double* st = (double*)hXBuf[k];
                 for (size_t r = 0; r < rows; ++r)
-                    for (int k = 0; k < kdim; ++k) st[r * kdim + k] = k < P ? loc[r * P + k] : 0.0;
+                    for (int c = 0; c < kdim; ++c) st[r * kdim + c] = c < P ? loc[r * P + c] : 0.0;
             }
-            check(hipMemcpyAsync((char*)dX[cur] + row0 * kdim * realSize, hX, rows * kdim * realSize,
-                                 hipMemcpyHostToDevice, stream),
+            check(hipMemcpyAsync((char*)full_write_target() + row0 * kdim * realSize, hXBuf[k],
+                                 rows * kdim * realSize, hipMemcpyHostToDevice, stream),
                   "hipMemcpy(locations)");
-            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            check(hipEventRecord(hXDone[k], stream), "hipEventRecord(locations)");
+            hXBusy[k] = true;
+            hXNext = 1 - k;
         }
         sumKnown = false;
         invalidate_device_results();
+    }
+
+    // The stored state moves out of dX[cur] before a write of part of it.
+    void resolve_store() {
+        if (!storePending) return;
+        check(hipMemcpyAsync(dX[1 - cur], dX[cur], (size_t)npad * kdim * realSize, hipMemcpyDeviceToDevice, stream),
+              "hipMemcpy(store)");
+        storePending = false;
+    }
+
+    // The buffer a write of every location goes to: with a store pending the
+    // other one, so that dX[cur] stays as the stored state without a copy.
+    void* full_write_target() {
+        if (storePending) {
+            cur = 1 - cur;
+            storePending = false;
+        }
+        return dX[cur];
     }
 
     // Writes the pending single-row update into the location array; the
     // O(N) path for it ends (its previous location is gone).
     void flush_row_write() {
         if (!rowWritePending) return;
+        resolve_store();
         RowWrite w;
         w.x = dX[cur];
         for (int k = 0; k < P; ++k) w.values[k] = rowWriteValues[k];
@@ -313,6 +355,7 @@ struct Engine {
     // The raw sum after the pending single-row update: base + half the
     // row's change, the block partials added in order.
     void applyRowUpdate() {
+        resolve_store();
         RowDeltaLaunch a;
         a.delta = dDelta;
         a.x = dX[cur];
@@ -414,20 +457,22 @@ struct Engine {
         storedSumValid = sumKnown && sumFromCurrentData;
         storedGradFresh = hostGradFresh;
         storedSharesCurrent = hostGradFresh;
-        check(hipMemcpyAsync(dX[1 - cur], dX[cur], (size_t)npad * kdim * realSize,
-                             hipMemcpyDeviceToDevice, stream),
-              "hipMemcpy(store)");
+        storePending = true;  // the copy is made only if dX[cur] is then written in part
         storedPrecision = precision;
     }
 
     void restoreState() {  // :234-255
-        flush_row_write();
+        // a pending single-row write belongs to the state given up: dropped
+        rowWritePending = false;
         sumOfIncrements = storedSumOfIncrements;
         sumKnown = true;
         sumFromCurrentData = storedSumValid;
         pendingRow = -1;
         precision = storedPrecision;
-        cur = 1 - cur;
+        if (storePending)
+            storePending = false;  // nothing was written since the store: dX[cur] is the stored state
+        else
+            cur = 1 - cur;
         invalidate_device_results();
         if (storedGradFresh) {
             if (!storedSharesCurrent) std::swap(hGrad, hGradStored);
@@ -537,17 +582,19 @@ struct Engine {
                     evaluate(predictGradient);
                     stateEval = predictGradient ? StateEval::Fused : StateEval::SumOnly;
                     if (predictGradient) {
+                        // the gradient and the sum stored after it, in one copy
                         own_host_gradient();
-                        check(hipMemcpyAsync(hGrad, dGrad, (size_t)N * P * sizeof(double), hipMemcpyDeviceToHost,
-                                             stream),
+                        check(hipMemcpyAsync(hGrad, dGrad, ((size_t)N * P + 1) * sizeof(double),
+                                             hipMemcpyDeviceToHost, stream),
                               "hipMemcpy(gradient)");
                         gradCopied = true;
                     }
                 }
-                check(hipMemcpyAsync(hSum, dSum, sizeof(double), hipMemcpyDeviceToHost, stream),
-                      "hipMemcpy(sum)");
+                if (!gradCopied)
+                    check(hipMemcpyAsync(hSum, dSum, sizeof(double), hipMemcpyDeviceToHost, stream),
+                          "hipMemcpy(sum)");
                 check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-                sumOfIncrements = *hSum;
+                sumOfIncrements = gradCopied ? hGrad[(size_t)N * P] : *hSum;
                 sumFromFullEvaluation();
                 if (gradCopied) hostGradFresh = true;
             }
@@ -571,16 +618,15 @@ struct Engine {
         }
         const bool fillSum = !sumKnown;
         own_host_gradient();
-        check(hipMemcpyAsync(hGrad, dGrad, length * sizeof(double), hipMemcpyDeviceToHost, stream),
+        // the gradient, and the sum stored after it when it is wanted too
+        check(hipMemcpyAsync(hGrad, dGrad, (length + (fillSum ? 1 : 0)) * sizeof(double), hipMemcpyDeviceToHost,
+                             stream),
               "hipMemcpy(gradient)");
-        if (fillSum)
-            check(hipMemcpyAsync(hSum, dSum, sizeof(double), hipMemcpyDeviceToHost, stream),
-                  "hipMemcpy(sum)");
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         std::memcpy(result, hGrad, length * sizeof(double));
         hostGradFresh = true;
         if (fillSum) {
-            sumOfIncrements = *hSum;
+            sumOfIncrements = hGrad[length];
             sumKnown = true;
             sumFromFullEvaluation();
         }

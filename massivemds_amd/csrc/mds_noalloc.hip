@@ -116,12 +116,13 @@ void evaluate_noalloc(Engine& e, const double* locations, double* sumOut, double
         e.pendingRow = -1;
         const int64_t count = (int64_t)e.N * e.kdim;
         const int blocks = (int)((count + kPackThreads - 1) / kPackThreads);
+        void* target = e.full_write_target();  // a pending stored state stays where it is
         if (e.fp32)
             hipLaunchKernelGGL(pack_locations_kernel<float>, dim3(blocks), dim3(kPackThreads), 0, e.stream, locations,
-                               (float*)e.dX[e.cur], e.N, e.P, e.kdim);
+                               (float*)target, e.N, e.P, e.kdim);
         else
             hipLaunchKernelGGL(pack_locations_kernel<double>, dim3(blocks), dim3(kPackThreads), 0, e.stream,
-                               locations, (double*)e.dX[e.cur], e.N, e.P, e.kdim);
+                               locations, (double*)target, e.N, e.P, e.kdim);
         check(hipGetLastError(), "location pack kernel");
         e.sumKnown = false;
         e.invalidate_device_results();
