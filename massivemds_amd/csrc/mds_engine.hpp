@@ -469,10 +469,10 @@ struct Engine {
         sumFromCurrentData = storedSumValid;
         pendingRow = -1;
         precision = storedPrecision;
-        if (storePending)
-            storePending = false;  // nothing was written since the store: dX[cur] is the stored state
-        else
-            cur = 1 - cur;
+        if (!storePending) cur = 1 - cur;  // else nothing was written since the store: dX[cur] is it
+        // the current locations are the stored ones again, and stay stored
+        // (a second restore keeps them, as the reference's copy does)
+        storePending = true;
         invalidate_device_results();
         if (storedGradFresh) {
             if (!storedSharesCurrent) std::swap(hGrad, hGradStored);

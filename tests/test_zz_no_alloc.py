@@ -115,6 +115,36 @@ def test_noalloc_matches_existing_entry_point(oracle):
         dev.free()
 
 
+def test_noalloc_locations_leave_a_stored_state_in_place(oracle):
+    """storeState, then new locations through the allocation-free entry, then
+    restoreState: the engine holds the stored locations again (evaluated
+    afresh: bitwise the existing entry point at those locations)."""
+    dev = _Device()
+    try:
+        tau = 1.1
+        e, delta = _instance(500, 2, True, False, tau, 77)
+        x1 = datagen.locations(500, 2, 78)
+        x2 = datagen.locations(500, 2, 79)
+        e.updateLocations(-1, x1)
+        e.getSumOfIncrements()
+        e.storeState()
+        s2, _ = _noalloc(dev, e, x2)
+        e.restoreState()
+        e.makeDirty()
+        s1 = e.getSumOfIncrements()
+        g1 = e.getLogLikelihoodGradient()
+        e.makeDirty()
+        s1_again, g1_again = _existing(e, x1)
+        assert s1 == s1_again and np.array_equal(g1, g1_again.ravel())
+        ref1 = oracle.sum_of_increments(delta, x1, tau, True)
+        ref2 = oracle.sum_of_increments(delta, x2, tau, True)
+        assert abs(s1 - ref1) <= 1e-12 * abs(ref1)
+        assert abs(s2 - ref2) <= 1e-12 * abs(ref2)
+        e.close()
+    finally:
+        dev.free()
+
+
 def test_noalloc_refuses_bad_buffers_and_sizes():
     dev = _Device()
     try:
