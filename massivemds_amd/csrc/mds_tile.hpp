@@ -131,18 +131,27 @@ struct TileAcc {
 // accumulators take half the pair coefficients (tau / 2 and sT / 2 in place
 // of tau and sT: exact halvings), and the tile doubles its partials at the
 // end, so that tau / 2 serves the likelihood and the gradient alike.
-template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
-__device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi)[P], const Real (&xj)[P],
-                                          Real dij, Real dji, bool inTri, const PairConst<Real>& pc) {
-    const Real halfTau = pc.halfTau;
-    const bool vij = inTri && !m_isnan(dij);
-    const bool vji = inTri && !m_isnan(dji);
+// pair_terms computes the step's terms for every lane; pair_accumulate adds
+// them for the observed pairs only.
+template <typename Real, int P>
+struct PairTerms {
     Real diff[P];
+    Real lij, lji, hij, hji;
+    bool vij, vji;
+};
+
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
+__device__ __forceinline__ PairTerms<Real, P> pair_terms(const Real (&xi)[P], const Real (&xj)[P], Real dij,
+                                                         Real dji, bool inTri, const PairConst<Real>& pc) {
+    PairTerms<Real, P> t;
+    const Real halfTau = pc.halfTau;
+    t.vij = inTri && !m_isnan(dij);
+    t.vji = inTri && !m_isnan(dji);
     Real d2 = Real(0);
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-        diff[p] = xi[p] - xj[p];
-        d2 = fma(diff[p], diff[p], d2);
+        t.diff[p] = xi[p] - xj[p];
+        d2 = fma(t.diff[p], t.diff[p], d2);
     }
     Real d, invd;
     distance(d2, d, invd);
@@ -150,36 +159,52 @@ __device__ __forceinline__ void pair_step(TileAcc<Real, P>& acc, const Real (&xi
     const Real rji = dji - d;
     const Real aij = halfTau * rij;
     const Real aji = halfTau * rji;
-
-    Real lij, lji;
     Real gt = Real(0);
     if (TRUNC) {
         const Trunc<Real> tt = trunc_terms(d, pc);
-        lij = fma(aij, rij, tt.logPhi);
-        lji = fma(aji, rji, tt.logPhi);
+        t.lij = fma(aij, rij, tt.logPhi);
+        t.lji = fma(aji, rji, tt.logPhi);
         gt = tt.gradTerm;
     } else {
-        lij = rij * rij;
-        lji = rji * rji;
+        t.lij = rij * rij;
+        t.lji = rji * rji;
     }
-    // symmetric data: each unordered pair counts twice; the factor 2 is
-    // applied to the tile's sum (exact, so the result is the same)
-    if (SYM)
-        acc.s.add(vij ? lij : Real(0));
-    else
-        acc.s.add((vij ? lij : Real(0)) + (vji ? lji : Real(0)));
+    // c / 2 = (r - phi / (Phi sqrt tau)) * (tau / 2) / d for each ordered
+    // pair (src/NewMultiDimensionalScaling.hpp:435-444)
+    t.hij = (TRUNC ? fma(-gt, pc.halfST, aij) : aij) * invd;
+    t.hji = SYM ? t.hij : (TRUNC ? fma(-gt, pc.halfST, aji) : aji) * invd;
+    return t;
+}
 
-    if (GRAD) {
-        // c / 2 = (r - phi / (Phi sqrt tau)) * (tau / 2) / d for each ordered
-        // pair (src/NewMultiDimensionalScaling.hpp:435-444); masked pairs give 0.
-        const Real hij = vij ? (TRUNC ? fma(-gt, pc.halfST, aij) : aij) * invd : Real(0);
-        const Real hji = SYM ? hij : (vji ? (TRUNC ? fma(-gt, pc.halfST, aji) : aji) * invd : Real(0));
+// Unobserved pairs add nothing: the additions run as branches on the
+// observation mask, so that the lanes of unobserved pairs are switched off
+// for them (no select per term), and the column accumulators rotate after.
+// Symmetric data: each unordered pair counts twice; the factor 2 is applied
+// to the tile's sum (exact, so the result is the same).
+template <typename Real, int P, bool SYM, bool GRAD>
+__device__ __forceinline__ void pair_accumulate(TileAcc<Real, P>& acc, const PairTerms<Real, P>& t) {
+    if (__builtin_expect(t.vij, 1)) {
+        acc.s.add(t.lij);
+        if (GRAD) {
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            acc.row[p].add_product(hij, diff[p]);
-            acc.col[p].add_product(SYM ? -hij : -hji, diff[p]);
-            acc.col[p].rotate();
+            for (int p = 0; p < P; ++p) {
+                acc.row[p].add_product(t.hij, t.diff[p]);
+                if (SYM) acc.col[p].add_product(-t.hij, t.diff[p]);
+            }
         }
+    }
+    if (!SYM) {
+        if (__builtin_expect(t.vji, 1)) {
+            acc.s.add(t.lji);
+            if (GRAD) {
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc.col[p].add_product(-t.hji, t.diff[p]);
+            }
+        }
+    }
+    if (GRAD) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc.col[p].rotate();
     }
 }
 
@@ -254,17 +279,31 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
 #pragma unroll
                 for (int p = 0; p < P; ++p) xs[g][p] = stageGroup[CS::at(g, p)];
         }
+        // FP32: the group's terms first (one block, in which the steps' work
+        // interleaves), then their additions step by step; FP64: each step's
+        // additions right after its terms (the compiler then moves the whole
+        // step under the observation mask).  Measured on the tile kernel at
+        // N = 10,000 against selects of zeros: FP32 -4 % (P = 2) to -7.5 %
+        // (P = 10) this way, +5 % the other; FP64 -2 % this way, +5 % the
+        // other (its terms of four steps at once contend for the LDS table).
+        constexpr bool kGroupTermsFirst = sizeof(Real) == 4;
+        PairTerms<Real, P> terms[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int k = k0 + g;
             const bool inTri = !DIAG || k < kTile / 2 || lane < kTile / 2;
             if constexpr (kStaged) {
-                pair_step<Real, P, TRUNC, SYM, GRAD>(acc, xi, xs[g], b[g], bT[g], inTri, pc);
+                terms[g] = pair_terms<Real, P, TRUNC, SYM, GRAD>(xi, xs[g], b[g], bT[g], inTri, pc);
             } else {
-                pair_step<Real, P, TRUNC, SYM, GRAD>(acc, xi, xj, b[g], bT[g], inTri, pc);
+                terms[g] = pair_terms<Real, P, TRUNC, SYM, GRAD>(xi, xj, b[g], bT[g], inTri, pc);
 #pragma unroll
                 for (int p = 0; p < P; ++p) xj[p] = dpp_rotate<kDppWaveRol1>(xj[p]);
             }
+            if constexpr (!kGroupTermsFirst) pair_accumulate<Real, P, SYM, GRAD>(acc, terms[g]);
+        }
+        if constexpr (kGroupTermsFirst) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) pair_accumulate<Real, P, SYM, GRAD>(acc, terms[g]);
         }
     };
 
