@@ -62,6 +62,7 @@ CASES = [
     (2, 2, "none"), (3, 1, "none"), (17, 2, "sym5"), (63, 3, "none"), (64, 2, "sym5"),
     (65, 2, "row"), (130, 5, "sym5"), (200, 2, "asym"), (257, 10, "sym5"), (129, 16, "asym"),
     (300, 4, "none"), (100, 6, "row"), (90, 8, "asym"), (70, 12, "none"), (1000, 2, "sym5"),
+    (150, 5, "asym"), (300, 3, "row"),
 ]
 
 
