@@ -24,18 +24,32 @@
 
 namespace mdship {
 
+// Wavefronts per workgroup of the single-instance tile kernel.  FP64 with
+// staged columns (P <= 2) runs eight, measured 2-4.5 % faster at
+// N = 5,000-20,000: with the 31 KB log Phi table per workgroup, LDS allows
+// three such workgroups per CU, up to 6 wavefronts per SIMD (what its 82
+// VGPRs allow) where four-wavefront workgroups allow 4, and the table is
+// copied once per eight tiles instead of four.  Everything else keeps four
+// (the FP32 kernels are held to 6 per SIMD by their registers with either;
+// the FP64 forms with P > 2 to 4).
+template <typename Real, int P>
+constexpr int tile_block_waves() {
+    return (sizeof(Real) == 8 && ColumnStage<Real, P>::kInLds) ? 2 * kWavesPerBlock : kWavesPerBlock;
+}
+
 // Single instance: one wavefront per tile of the upper block triangle.  The
 // truncated forms copy the log Phi table into LDS and every wavefront stages
 // its tile's column locations (small P); all wavefronts of the block take
 // part in the barrier before any leaves.
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
-__global__ __launch_bounds__(kBlockThreads) void tile_kernel(
+__global__ __launch_bounds__((tile_block_waves<Real, P>() * kWave)) void tile_kernel(
     const Real* __restrict__ delta, const Real* __restrict__ x,
     const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
     double tau, double* __restrict__ tileSums, Real* __restrict__ part, const double* __restrict__ devTau) {
+    constexpr int kWaves = tile_block_waves<Real, P>();
     __shared__ __align__(16) Real lnPhiTab[TRUNC ? TableOf<Real>::kCount : 1];
-    __shared__ __align__(16) Real colStage[kWavesPerBlock][ColumnStage<Real, P>::kElems];
-    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave);
+    __shared__ __align__(16) Real colStage[kWaves][ColumnStage<Real, P>::kElems];
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + threadIdx.x / kWave);
     const uint32_t code = t < numTiles ? tiles[t] : 0u;
     Real* stage = colStage[threadIdx.x / kWave];
     if (t < numTiles) stage_columns<Real, P>(stage, x, (int)(code & 0xffffu));
@@ -134,8 +148,9 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
 
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 static hipError_t go(const TileLaunch& a, hipStream_t s) {
-    const dim3 grid((a.numTiles + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((tile_kernel<Real, P, TRUNC, SYM, GRAD>), grid, dim3(kBlockThreads), 0, s,
+    constexpr int kWaves = tile_block_waves<Real, P>();
+    const dim3 grid((a.numTiles + kWaves - 1) / kWaves);
+    hipLaunchKernelGGL((tile_kernel<Real, P, TRUNC, SYM, GRAD>), grid, dim3(kWaves * kWave), 0, s,
                        (const Real*)a.delta, (const Real*)a.x, a.tiles, a.numTiles, a.nb, a.planeStride, a.tau,
                        a.tileSums, (Real*)a.part, a.devTau);
     return hipGetLastError();
