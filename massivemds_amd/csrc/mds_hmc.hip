@@ -196,7 +196,7 @@ Hmc::Hmc(Engine& engine, const mds_hip_hmc_options& options) : e(engine), opt(op
     check(hipMalloc((void**)&dParams, kHmcParamCount * sizeof(double)), "hipMalloc(parameters)");
     check(hipMalloc((void**)&dEnergy, 2 * sizeof(double)), "hipMalloc(energy)");
     check(hipHostMalloc((void**)&hParams, kHmcParamCount * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-    check(hipHostMalloc((void**)&hScalars, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+    check(hipHostMalloc((void**)&hScalars, 5 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
     check(hipHostMalloc(&hX, xBytes, hipHostMallocDefault), "hipHostMalloc");
     check(hipMalloc((void**)&dRows, kTraceRows * kMaxDim * sizeof(double)), "hipMalloc(trace rows)");
     check(hipMalloc((void**)&dGram, kMaxDim * kMaxDim * sizeof(double)), "hipMalloc(gram)");
@@ -338,8 +338,14 @@ void Hmc::buildGraph(int mode) {
 }
 
 // energies at the current locations: hScalars[0] = K, [1] = x^T T x sum,
-// [2] = sumOfIncrements (0.5 * sum of tile partials)
+// [2] = sumOfIncrements (0.5 * sum of tile partials); enqueueEnergies leaves
+// the wait to the caller
 void Hmc::readEnergies() {
+    enqueueEnergies();
+    check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+}
+
+void Hmc::enqueueEnergies() {
     if (e.fp32)
         hipLaunchKernelGGL((energy_kernel<float>), dim3(1), dim3(kEnergyThreads), 0, e.stream,
                            (const float*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, dEnergy);
@@ -349,7 +355,24 @@ void Hmc::readEnergies() {
     check(hipGetLastError(), "energy kernel");
     check(hipMemcpyAsync(hScalars, dEnergy, 2 * sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
     check(hipMemcpyAsync(hScalars + 2, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
-    check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+}
+
+// The work the rest of an iteration does at the locations the HMC step
+// keeps, enqueued ahead of the wait on the HMC step's energies: the trace
+// rows (record), X^T X (trait step) and, for tauStar > 0, the likelihood at
+// the precision proposal tauStar into hScalars[4] (the device parameter block
+// holds tauStar afterwards; the host's tau is left as it was).
+void Hmc::enqueueAtKept(bool traitStep, bool recordIt, double tauStar) {
+    if (traitStep) gatherGram();
+    if (recordIt) gatherTraceRows();
+    if (tauStar > 0.0) {
+        const double saved = tau;
+        tau = tauStar;
+        uploadParams();
+        tau = saved;
+        e.evaluate(false, dParams + kHmcTau);
+        check(hipMemcpyAsync(hScalars + 4, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
+    }
 }
 
 double mds_log_likelihood(double rawSum, double t, int n, bool truncated) {
@@ -519,14 +542,36 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             evaluations += opt.leapfrogSteps;
             ++exactIts;
         }
-        readEnergies();
+        enqueueEnergies();
+        // the trace rows and X^T X of the kept locations (the precision step
+        // does not move them)
+        const bool traitStep = opt.learnTraitPrecision && it < opt.iterations - 1 &&
+                               ((it + 1) % opt.traitInvWeight == 0);
+        const bool recordIt = it >= opt.burnIn && trace;
+        // The random draws up to the precision's Metropolis test do not depend
+        // on the device's results, so they are made here, in the sampler's
+        // order (the HMC test's uniform, the precision proposal, its test's
+        // uniform), and with exact gradients and a truncated likelihood the
+        // rest of the iteration's device work is enqueued at the proposed
+        // locations before the one wait: kept in ~93 % of iterations, redone
+        // at the restored locations otherwise.  Every value is the one the
+        // steps would give in turn.
+        const bool speculate = mode != kSurrogate && opt.learnPrecision && e.truncated;
+        const double u = rng.uniform();
+        double tauStar = 0.0, uTau = 0.0;
+        if (speculate) {
+            const double lo = tau - width.w, hi = tau + width.w;
+            tauStar = std::fabs(lo + (hi - lo) * rng.uniform());
+            if (tauStar > 0.0) uTau = rng.uniform();
+            enqueueAtKept(traitStep, recordIt, tauStar);
+        }
+        check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
         const double K0 = hScalars[3];
         const double K1 = hScalars[0];
         const double logLik1 = logLikelihood(hScalars[2], tau);
         const double quad1 = hScalars[1];
         const double U1 = -(logLik1 + logPrior(quad1));
         const double ratio = (U + K0) - (U1 + K1);  // R/Untitled.R:630-637
-        const double u = rng.uniform();
         const bool acc = std::isfinite(ratio) && ratio > std::min(0.0, std::log(u));
         if (acc && mode == kSurrogate) ++surrogateAccepted;
         if (acc) {
@@ -540,6 +585,10 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             if (mode == kExactCollect)
                 check(hipMemcpyAsync(sur->store.dSlot, sur->store.dSlot + 1, sizeof(double), hipMemcpyDeviceToDevice,
                                      e.stream), "hipMemcpy(slot)");
+            if (speculate) {
+                enqueueAtKept(traitStep, recordIt, tauStar);
+                check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+            }
         }
         if (opt.adaptStepSize && it < opt.burnIn) {
             const double alpha = std::isfinite(ratio) ? std::min(1.0, std::exp(std::min(ratio, 0.0))) : 0.0;
@@ -571,15 +620,34 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             }
         }
 
-        // the trace rows and X^T X of the kept locations (the precision step
-        // does not move them), read after the next wait
-        const bool traitStep = opt.learnTraitPrecision && it < opt.iterations - 1 &&
-                               ((it + 1) % opt.traitInvWeight == 0);
-        const bool recordIt = it >= opt.burnIn && trace;
-        if (traitStep) gatherGram();
-        if (recordIt) gatherTraceRows();
+        if (!speculate) {  // read after the next wait
+            if (traitStep) gatherGram();
+            if (recordIt) gatherTraceRows();
+        }
 
-        if (opt.learnPrecision) {
+        if (speculate) {
+            // random-walk Metropolis at the kept locations, from the pass made
+            // ahead (its likelihood in hScalars[4])
+            ++proposedTau;
+            if (tauStar > 0.0) {
+                ++evaluations;
+                const double logLikStar = logLikelihood(hScalars[4], tauStar);
+                const double r = logLikStar - logLik;  // the prior does not involve tau
+                const bool accTau = std::isfinite(r) && r > std::min(0.0, std::log(uTau));
+                width.update(accTau, it < opt.burnIn);
+                if (accTau) {
+                    tau = tauStar;
+                    logLik = logLikStar;
+                    U = -(logLik + logPrior(quad));
+                    ++acceptedTau;
+                    uploadParams();
+                    gradientAtCurrent();  // gradient at the new tau
+                    ++evaluations;
+                } else {
+                    uploadParams();  // the device block held tauStar for the pass
+                }
+            }
+        } else if (opt.learnPrecision) {
             if (gradMode == kSurrogate) {
                 gradient(exactMode);  // precision moves use the exact likelihood and gradient
                 gradMode = exactMode;
@@ -637,7 +705,7 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             }
         }
 
-        if (traitStep || recordIt) check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
+        if (!speculate && (traitStep || recordIt)) check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
         if (traitStep) {
             std::vector<double> m((size_t)p * p, 0.0);  // T0 + X^T X, T0 = I_P
             for (int d = 0; d < p; ++d) m[d * p + d] = 1.0;

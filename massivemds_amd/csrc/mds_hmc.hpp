@@ -104,6 +104,8 @@ struct Hmc {
     double logPrior(double quad) const;
     void copyLocationsToHost(std::vector<double>& out);
     void gatherTraceRows();
+    void enqueueEnergies();
+    void enqueueAtKept(bool traitStep, bool recordIt, double tauStar);
     void gatherGram();
     void record(double* row, const double* rows, double logPost) const;
     void run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats* stats);
