@@ -941,10 +941,14 @@ struct BatchHmc {
             }
             check(hipMemcpy(dChains, desc.data(), (size_t)C * sizeof(ChainDesc), hipMemcpyHostToDevice),
                   "hipMemcpy(chains)");
-            // one tile launch over every chain (no side streams: the trajectory
-            // is captured on the batch's stream), the precision read from each
-            // chain's parameter block
-            b.cfg.strategy = 0;
+            // one tile launch over every chain, on the batch's stream (no side
+            // streams: the trajectory is captured there): the kernel of the
+            // chains' one embedding-dimension class when they share it (its
+            // registers are that class's alone), else the kernel for every
+            // class; the precision read from each chain's parameter block
+            bool oneClass = true;
+            for (int c = 1; c < C; ++c) oneClass = oneClass && b.e[c].kdim == b.e[0].kdim;
+            b.cfg.strategy = oneClass ? 1 : 0;
             for (int c = 0; c < C; ++c) {
                 b.e[c].devTau = dParams + (size_t)c * kHmcParamCount + kHmcTau;
                 b.e[c].kickMom = dMom + ch[c].momOffset;
