@@ -497,8 +497,8 @@ struct Engine {
     // the given tile-sum and partial buffers.  devTau: when non-null, the
     // kernels read the precision from this device address instead of the
     // engine's value (for captured graphs whose precision changes between
-    // replays).
-    void launch_tile_pass(bool gradient, const double* devTau, double* tileSums, void* part) {
+    // replays); otherwise they take tau.
+    void launch_tile_pass(bool gradient, const double* devTau, double* tileSums, void* part, double tau) {
         TileLaunch a;
         a.delta = dDelta;
         a.x = dX[cur];
@@ -507,7 +507,7 @@ struct Engine {
         a.nb = nb;
         a.kdim = kdim;
         a.planeStride = (int64_t)numTiles * kTileElems;
-        a.tau = precision;
+        a.tau = tau;
         a.truncated = truncated;
         a.symmetric = symmetric;
         a.gradient = gradient;
@@ -543,7 +543,12 @@ struct Engine {
         check(launch_reduce(r, stream), "reduce kernel");
     }
 
-    void evaluate(bool gradient, const double* devTau = nullptr, const ReduceKick& kick = ReduceKick()) {
+    // One fused pass at the current locations.  tauValue > 0: at that
+    // precision instead of the engine's (when devTau is null); sumOut: the
+    // sum goes there (a device-visible address, pinned host memory included)
+    // instead of dSum, which then keeps its previous value.
+    void evaluate(bool gradient, const double* devTau = nullptr, const ReduceKick& kick = ReduceKick(),
+                  double tauValue = 0.0, double* sumOut = nullptr) {
         if (!dataSet) throw Error(MDS_HIP_ERR_STATE, "pairwise data not set");
         if (!precisionSet) throw Error(MDS_HIP_ERR_STATE, "precision not set");
         flush_row_write();
@@ -554,17 +559,18 @@ struct Engine {
             e2 = take_event();
             check(hipEventRecord(e0, stream), "hipEventRecord");
         }
-        launch_tile_pass(gradient, devTau, dTileSums, dPart);
+        launch_tile_pass(gradient, devTau, dTileSums, dPart, tauValue > 0.0 ? tauValue : precision);
         if (timing) check(hipEventRecord(e1, stream), "hipEventRecord");
-        launch_reduce_pass(gradient, dTileSums, dPart, dGrad, dSum, 1.0, kick);
+        launch_reduce_pass(gradient, dTileSums, dPart, dGrad, sumOut ? sumOut : dSum, 1.0, kick);
         if (timing) {
             check(hipEventRecord(e2, stream), "hipEventRecord");
             pending.push_back(e0);
             pending.push_back(e1);
             pending.push_back(e2);
         }
-        devSumFresh = true;
-        if (gradient) gradKnown = true;
+        const bool ownTau = tauValue <= 0.0;
+        if (!sumOut) devSumFresh = ownTau;  // dSum rewritten
+        if (gradient) gradKnown = ownTau;   // dGrad rewritten
         hostGradFresh = false;
         ++fullEvaluations;
     }

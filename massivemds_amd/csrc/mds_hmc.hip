@@ -28,6 +28,7 @@
 // rejection.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -44,10 +45,27 @@ namespace mdship {
 
 // ------------------------------------------------------------------ kernels
 
-// Standard normal momenta from a counter-based stream keyed by (seed,
-// iteration), one pair per thread (momentum_pair).
-__global__ void momentum_kernel(double* __restrict__ mom, int count, uint64_t seed, uint64_t iteration) {
-    momentum_pair(mom, count, seed, iteration, blockIdx.x * blockDim.x + threadIdx.x);
+// The start of an iteration in one launch: standard normal momenta from a
+// counter-based stream keyed by (seed, iteration), one pair per thread
+// (momentum_pair), and copies of the locations and of the gradient the
+// trajectory starts from (put back after a rejected move).
+template <typename Real>
+__global__ void start_kernel(double* __restrict__ mom, int count, uint64_t seed, uint64_t iteration,
+                             const Real* __restrict__ x, Real* __restrict__ xSave, int64_t xCount,
+                             const double* __restrict__ grad, double* __restrict__ gradSave, int64_t gCount) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    momentum_pair(mom, count, seed, iteration, (int)t);
+    if (t < xCount) xSave[t] = x[t];
+    if (t < gCount) gradSave[t] = grad[t];
+}
+
+// The copies back after a rejected move.
+template <typename Real>
+__global__ void restore_kernel(const Real* __restrict__ xSave, Real* __restrict__ x, int64_t xCount,
+                               const double* __restrict__ gradSave, double* __restrict__ grad, int64_t gCount) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < xCount) x[t] = xSave[t];
+    if (t < gCount) grad[t] = gradSave[t];
 }
 
 template <typename Real, int FORM>
@@ -56,13 +74,18 @@ __global__ void kick_kernel(Real* __restrict__ x, double* __restrict__ mom, cons
     kick_row<Real, FORM>(x, mom, grad, params, n, p, kdim, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// out[0..1] as energy_block; with sumIn, also *sumOut = *sumIn (the sum of
+// the pass before, carried to the host with the energies).
 template <typename Real>
 __global__ __launch_bounds__(kEnergyThreads) void energy_kernel(const Real* __restrict__ x,
                                                                 const double* __restrict__ mom,
                                                                 const double* __restrict__ params, int n, int p,
-                                                                int kdim, double* __restrict__ out) {
+                                                                int kdim, double* __restrict__ out,
+                                                                const double* __restrict__ sumIn,
+                                                                double* __restrict__ sumOut) {
     __shared__ double red[2][kEnergyThreads];
     energy_block<Real>(x, mom, params, n, p, kdim, out, red);
+    if (sumIn && threadIdx.x == 0) *sumOut = *sumIn;
 }
 
 template <typename Real>
@@ -194,14 +217,16 @@ Hmc::Hmc(Engine& engine, const mds_hip_hmc_options& options) : e(engine), opt(op
     check(hipMalloc((void**)&dGradSave, np * sizeof(double)), "hipMalloc(gradient copy)");
     check(hipMalloc(&dXSave, xBytes), "hipMalloc(location copy)");
     check(hipMalloc((void**)&dParams, kHmcParamCount * sizeof(double)), "hipMalloc(parameters)");
-    check(hipMalloc((void**)&dEnergy, 2 * sizeof(double)), "hipMalloc(energy)");
     check(hipHostMalloc((void**)&hParams, kHmcParamCount * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-    check(hipHostMalloc((void**)&hScalars, 5 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
     check(hipHostMalloc(&hX, xBytes, hipHostMallocDefault), "hipHostMalloc");
-    check(hipMalloc((void**)&dRows, kTraceRows * kMaxDim * sizeof(double)), "hipMalloc(trace rows)");
-    check(hipMalloc((void**)&dGram, kMaxDim * kMaxDim * sizeof(double)), "hipMalloc(gram)");
-    check(hipHostMalloc((void**)&hRows, kTraceRows * kMaxDim * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-    check(hipHostMalloc((void**)&hGram, kMaxDim * kMaxDim * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+    // written by kernels (coherent pinned memory, read after a stream wait)
+    check(hipHostMalloc((void**)&hScalars, kScalarSlots * sizeof(double), hipHostMallocCoherent), "hipHostMalloc");
+    check(hipHostMalloc((void**)&hRows, kTraceRows * kMaxDim * sizeof(double), hipHostMallocCoherent),
+          "hipHostMalloc");
+    check(hipHostMalloc((void**)&hGram, kMaxDim * kMaxDim * sizeof(double), hipHostMallocCoherent), "hipHostMalloc");
+    check(hipHostGetDevicePointer((void**)&kScalars, hScalars, 0), "hipHostGetDevicePointer");
+    check(hipHostGetDevicePointer((void**)&kRows, hRows, 0), "hipHostGetDevicePointer");
+    check(hipHostGetDevicePointer((void**)&kGram, hGram, 0), "hipHostGetDevicePointer");
     check(hipEventCreateWithFlags(&uploadDone, hipEventDisableTiming), "hipEventCreate");
     traceRows = trace_rows(n);
     std::memset(hParams, 0, kHmcParamCount * sizeof(double));
@@ -229,12 +254,9 @@ Hmc::~Hmc() {
     (void)hipFree(dGradSave);
     (void)hipFree(dXSave);
     (void)hipFree(dParams);
-    (void)hipFree(dEnergy);
     (void)hipHostFree(hParams);
     (void)hipHostFree(hScalars);
     (void)hipHostFree(hX);
-    (void)hipFree(dRows);
-    (void)hipFree(dGram);
     (void)hipHostFree(hRows);
     (void)hipHostFree(hGram);
     if (uploadDone) (void)hipEventDestroy(uploadDone);
@@ -244,6 +266,7 @@ Hmc::~Hmc() {
 // run (the copy reads host memory when it executes, not when it is queued).
 void Hmc::uploadParams() {
     if (uploadPending) check(hipEventSynchronize(uploadDone), "hipEventSynchronize(parameters)");
+    paramsStale = false;
     hParams[kHmcTau] = tau;
     hParams[kHmcEps] = eps;
     for (int k = 0; k < p * p; ++k) hParams[kHmcTrait + k] = trait[k];
@@ -337,42 +360,40 @@ void Hmc::buildGraph(int mode) {
     check(hipGraphInstantiate(&graphExec[mode], graph[mode], nullptr, nullptr, 0), "hipGraphInstantiate");
 }
 
-// energies at the current locations: hScalars[0] = K, [1] = x^T T x sum,
-// [2] = sumOfIncrements (0.5 * sum of tile partials); enqueueEnergies leaves
-// the wait to the caller
+// energies at the current locations: hScalars[kSlotK] = K, [kSlotQuad] =
+// x^T T x sum, [kSlotSum] = the last pass's sumOfIncrements (0.5 * sum of
+// tile partials); enqueueEnergies leaves the wait to the caller
 void Hmc::readEnergies() {
     enqueueEnergies();
     check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
 }
 
-void Hmc::enqueueEnergies() {
+void Hmc::enqueueEnergies() { launchEnergy(kSlotK, true); }
+
+// The energy kernel at the current locations and momenta into the slots
+// from `slot` on; withSum: the last pass's sum into kSlotSum as well.
+void Hmc::launchEnergy(int slot, bool withSum) {
+    const double* sumIn = withSum ? e.dSum : nullptr;
     if (e.fp32)
         hipLaunchKernelGGL((energy_kernel<float>), dim3(1), dim3(kEnergyThreads), 0, e.stream,
-                           (const float*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, dEnergy);
+                           (const float*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, kScalars + slot, sumIn,
+                           kScalars + kSlotSum);
     else
         hipLaunchKernelGGL((energy_kernel<double>), dim3(1), dim3(kEnergyThreads), 0, e.stream,
-                           (const double*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, dEnergy);
+                           (const double*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, kScalars + slot, sumIn,
+                           kScalars + kSlotSum);
     check(hipGetLastError(), "energy kernel");
-    check(hipMemcpyAsync(hScalars, dEnergy, 2 * sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
-    check(hipMemcpyAsync(hScalars + 2, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
 }
 
 // The work the rest of an iteration does at the locations the HMC step
 // keeps, enqueued ahead of the wait on the HMC step's energies: the trace
 // rows (record), X^T X (trait step) and, for tauStar > 0, the likelihood at
-// the precision proposal tauStar into hScalars[4] (the device parameter block
-// holds tauStar afterwards; the host's tau is left as it was).
+// the precision proposal tauStar into hScalars[kSlotSumStar] (tauStar passed
+// to the kernels by value: the device parameter block keeps tau).
 void Hmc::enqueueAtKept(bool traitStep, bool recordIt, double tauStar) {
     if (traitStep) gatherGram();
     if (recordIt) gatherTraceRows();
-    if (tauStar > 0.0) {
-        const double saved = tau;
-        tau = tauStar;
-        uploadParams();
-        tau = saved;
-        e.evaluate(false, dParams + kHmcTau);
-        check(hipMemcpyAsync(hScalars + 4, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
-    }
+    if (tauStar > 0.0) e.evaluate(false, nullptr, ReduceKick(), tauStar, kScalars + kSlotSumStar);
 }
 
 double mds_log_likelihood(double rawSum, double t, int n, bool truncated) {
@@ -392,28 +413,56 @@ double Hmc::logLikelihood(double rawSum, double t) const { return mds_log_likeli
 double Hmc::logPrior(double quad) const { return matrix_normal_log_prior(quad, n, p, trait); }
 
 // Enqueues the trace rows (record) and X^T X (trait) of the current
-// locations into pinned host memory; the caller synchronises before reading.
+// locations, written by the kernels into pinned host memory; the caller
+// synchronises before reading.
 void Hmc::gatherTraceRows() {
     if (e.fp32)
         hipLaunchKernelGGL(gather_rows_kernel<float>, dim3(1), dim3(kTraceRows * kMaxDim), 0, e.stream,
-                           (const float*)e.dX[e.cur], e.kdim, p, traceRows, dRows);
+                           (const float*)e.dX[e.cur], e.kdim, p, traceRows, kRows);
     else
         hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(1), dim3(kTraceRows * kMaxDim), 0, e.stream,
-                           (const double*)e.dX[e.cur], e.kdim, p, traceRows, dRows);
+                           (const double*)e.dX[e.cur], e.kdim, p, traceRows, kRows);
     check(hipGetLastError(), "trace rows kernel");
-    check(hipMemcpyAsync(hRows, dRows, kTraceRows * p * sizeof(double), hipMemcpyDeviceToHost, e.stream),
-          "hipMemcpy(trace rows)");
+}
+
+void Hmc::startIteration(int it) {
+    const int count = n * p;
+    const int64_t xCount = (int64_t)e.npad * e.kdim, gCount = (int64_t)n * p;
+    const int64_t work = std::max<int64_t>(std::max<int64_t>((count + 1) / 2, xCount), gCount);
+    constexpr int kThreads = 256;
+    const dim3 grid((unsigned)((work + kThreads - 1) / kThreads));
+    if (e.fp32)
+        hipLaunchKernelGGL(start_kernel<float>, grid, dim3(kThreads), 0, e.stream, dMom, count, (uint64_t)opt.seed,
+                           (uint64_t)it, (const float*)e.dX[e.cur], (float*)dXSave, xCount, e.dGrad, dGradSave,
+                           gCount);
+    else
+        hipLaunchKernelGGL(start_kernel<double>, grid, dim3(kThreads), 0, e.stream, dMom, count, (uint64_t)opt.seed,
+                           (uint64_t)it, (const double*)e.dX[e.cur], (double*)dXSave, xCount, e.dGrad, dGradSave,
+                           gCount);
+    check(hipGetLastError(), "start kernel");
+}
+
+void Hmc::restoreStart() {
+    const int64_t xCount = (int64_t)e.npad * e.kdim, gCount = (int64_t)n * p;
+    constexpr int kThreads = 256;
+    const dim3 grid((unsigned)((std::max(xCount, gCount) + kThreads - 1) / kThreads));
+    if (e.fp32)
+        hipLaunchKernelGGL(restore_kernel<float>, grid, dim3(kThreads), 0, e.stream, (const float*)dXSave,
+                           (float*)e.dX[e.cur], xCount, dGradSave, e.dGrad, gCount);
+    else
+        hipLaunchKernelGGL(restore_kernel<double>, grid, dim3(kThreads), 0, e.stream, (const double*)dXSave,
+                           (double*)e.dX[e.cur], xCount, dGradSave, e.dGrad, gCount);
+    check(hipGetLastError(), "restore kernel");
 }
 
 void Hmc::gatherGram() {
     if (e.fp32)
         hipLaunchKernelGGL(gram_kernel<float>, dim3(p * p), dim3(kGramThreads), 0, e.stream, (const float*)e.dX[e.cur],
-                           n, e.kdim, p, dGram);
+                           n, e.kdim, p, kGram);
     else
         hipLaunchKernelGGL(gram_kernel<double>, dim3(p * p), dim3(kGramThreads), 0, e.stream,
-                           (const double*)e.dX[e.cur], n, e.kdim, p, dGram);
+                           (const double*)e.dX[e.cur], n, e.kdim, p, kGram);
     check(hipGetLastError(), "gram kernel");
-    check(hipMemcpyAsync(hGram, dGram, p * p * sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy(gram)");
 }
 
 void Hmc::copyLocationsToHost(std::vector<double>& out) {
@@ -477,16 +526,14 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
     gradient(exactMode);
     check(hipMemsetAsync(dMom, 0, (size_t)n * p * sizeof(double), e.stream), "hipMemset(momentum)");
     readEnergies();
-    double logLik = logLikelihood(hScalars[2], tau);
-    double quad = hScalars[1];
+    double logLik = logLikelihood(hScalars[kSlotSum], tau);
+    double quad = hScalars[kSlotQuad];
     double U = -(logLik + logPrior(quad));
 
     // dual averaging of the step size (Hoffman & Gelman 2014, sec. 3.2)
     const double daMu = std::log(10.0 * eps), daGamma = 0.05, daT0 = 10.0, daKappa = 0.75;
     double daH = 0.0, daLogEpsBar = 0.0;
 
-    const size_t xBytes = (size_t)e.npad * e.kdim * e.realSize;
-    const size_t gBytes = (size_t)n * p * sizeof(double);
     long long accepted = 0, proposedTau = 0, acceptedTau = 0, evaluations = 1;
     long long surrogateIts = 0, exactIts = 0, surrogateAccepted = 0;
     int windowIts = 0, windowAcc = 0, fallbackLeft = 0, failedWindows = 0;
@@ -494,7 +541,6 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
     width.w = opt.precisionProposalWidth;
     width.adapt = opt.adaptPrecisionWidth != 0;
     auto tSampling = clock::now();
-    const int threads = 256;
     for (int it = 0; it < opt.iterations; ++it) {
         if (it == opt.burnIn) {
             tSampling = clock::now();
@@ -510,22 +556,11 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
         }
         gradMode = mode;
         if (opt.useGraph) buildGraph(mode);
-        const int count = n * p;
-        hipLaunchKernelGGL(momentum_kernel, dim3((count / 2 + threads) / threads), dim3(threads), 0, e.stream, dMom,
-                           count, (uint64_t)opt.seed, (uint64_t)it);
-        check(hipGetLastError(), "momentum kernel");
-        // the starting kinetic energy is read back with the energies after the
-        // trajectory (no wait here)
-        if (e.fp32)
-            hipLaunchKernelGGL((energy_kernel<float>), dim3(1), dim3(kEnergyThreads), 0, e.stream,
-                               (const float*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, dEnergy);
-        else
-            hipLaunchKernelGGL((energy_kernel<double>), dim3(1), dim3(kEnergyThreads), 0, e.stream,
-                               (const double*)e.dX[e.cur], dMom, dParams, n, p, e.kdim, dEnergy);
-        check(hipGetLastError(), "energy kernel");
-        check(hipMemcpyAsync(hScalars + 3, dEnergy, sizeof(double), hipMemcpyDeviceToHost, e.stream), "hipMemcpy");
-        check(hipMemcpyAsync(dXSave, e.dX[e.cur], xBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
-        check(hipMemcpyAsync(dGradSave, e.dGrad, gBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
+        // the momenta and the copies a rejection restores, then the starting
+        // kinetic energy, read with the energies after the trajectory (no
+        // wait here)
+        startIteration(it);
+        launchEnergy(kSlotK0, false);
         if (mode == kExactCollect)  // snapshots of a rejected trajectory are dropped
             check(hipMemcpyAsync(sur->store.dSlot + 1, sur->store.dSlot, sizeof(double), hipMemcpyDeviceToDevice,
                                  e.stream), "hipMemcpy(slot)");
@@ -566,10 +601,10 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             enqueueAtKept(traitStep, recordIt, tauStar);
         }
         check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
-        const double K0 = hScalars[3];
-        const double K1 = hScalars[0];
-        const double logLik1 = logLikelihood(hScalars[2], tau);
-        const double quad1 = hScalars[1];
+        const double K0 = hScalars[kSlotK0];
+        const double K1 = hScalars[kSlotK];
+        const double logLik1 = logLikelihood(hScalars[kSlotSum], tau);
+        const double quad1 = hScalars[kSlotQuad];
         const double U1 = -(logLik1 + logPrior(quad1));
         const double ratio = (U + K0) - (U1 + K1);  // R/Untitled.R:630-637
         const bool acc = std::isfinite(ratio) && ratio > std::min(0.0, std::log(u));
@@ -580,8 +615,7 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             quad = quad1;
             ++accepted;
         } else {
-            check(hipMemcpyAsync(e.dX[e.cur], dXSave, xBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
-            check(hipMemcpyAsync(e.dGrad, dGradSave, gBytes, hipMemcpyDeviceToDevice, e.stream), "hipMemcpy");
+            restoreStart();
             if (mode == kExactCollect)
                 check(hipMemcpyAsync(sur->store.dSlot, sur->store.dSlot + 1, sizeof(double), hipMemcpyDeviceToDevice,
                                      e.stream), "hipMemcpy(slot)");
@@ -627,11 +661,11 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
 
         if (speculate) {
             // random-walk Metropolis at the kept locations, from the pass made
-            // ahead (its likelihood in hScalars[4])
+            // ahead (its likelihood in hScalars[kSlotSumStar])
             ++proposedTau;
             if (tauStar > 0.0) {
                 ++evaluations;
-                const double logLikStar = logLikelihood(hScalars[4], tauStar);
+                const double logLikStar = logLikelihood(hScalars[kSlotSumStar], tauStar);
                 const double r = logLikStar - logLik;  // the prior does not involve tau
                 const bool accTau = std::isfinite(r) && r > std::min(0.0, std::log(uTau));
                 width.update(accTau, it < opt.burnIn);
@@ -640,11 +674,11 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
                     logLik = logLikStar;
                     U = -(logLik + logPrior(quad));
                     ++acceptedTau;
-                    uploadParams();
-                    gradientAtCurrent();  // gradient at the new tau
+                    // the gradient at the new tau, passed by value; the device
+                    // parameter block follows at the end of the iteration
+                    e.evaluate(true, nullptr, ReduceKick(), tau);
+                    paramsStale = true;
                     ++evaluations;
-                } else {
-                    uploadParams();  // the device block held tauStar for the pass
                 }
             }
         } else if (opt.learnPrecision) {
@@ -664,10 +698,11 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
                     uploadParams();
                     e.evaluate(false, dParams + kHmcTau);
                     ++evaluations;
-                    check(hipMemcpyAsync(hScalars + 2, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream),
+                    check(hipMemcpyAsync(hScalars + kSlotSum, e.dSum, sizeof(double), hipMemcpyDeviceToHost,
+                                         e.stream),
                           "hipMemcpy");
                     check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
-                    const double logLikStar = logLikelihood(hScalars[2], tauStar);
+                    const double logLikStar = logLikelihood(hScalars[kSlotSum], tauStar);
                     const double r = logLikStar - logLik;  // the prior does not involve tau
                     const bool accTau = std::isfinite(r) && r > std::min(0.0, std::log(rng.uniform()));
                     width.update(accTau, it < opt.burnIn);
@@ -689,10 +724,10 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
                 // sum of squared residuals (this engine's raw sum at the current X)
                 gradientAtCurrent();
                 ++evaluations;
-                check(hipMemcpyAsync(hScalars + 2, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream),
+                check(hipMemcpyAsync(hScalars + kSlotSum, e.dSum, sizeof(double), hipMemcpyDeviceToHost, e.stream),
                       "hipMemcpy");
                 check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");
-                const double C = hScalars[2];
+                const double C = hScalars[kSlotSum];
                 const double pairs = (double)n * (n - 1) / 2.0;
                 tau = gamma_draw(rng, 1.0 + 0.5 * pairs) / (0.5 * C);
                 ++proposedTau;
@@ -717,6 +752,7 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
             for (int k = 0; k < p * p; ++k) quad += trait[k] * hGram[k];
             U = -(logLik + logPrior(quad));
         }
+        if (paramsStale) uploadParams();  // an accepted precision without a trait step
         if (recordIt) record(trace + (size_t)(it - opt.burnIn) * kHmcTraceColumns, hRows, -U);
     }
     check(hipStreamSynchronize(e.stream), "hipStreamSynchronize");

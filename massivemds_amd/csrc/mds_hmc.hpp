@@ -73,16 +73,22 @@ struct Hmc {
     double* dGradSave = nullptr;
     void* dXSave = nullptr;
     double* dParams = nullptr;
-    double* dEnergy = nullptr;
     double* hParams = nullptr;
     hipEvent_t uploadDone = nullptr;  // after the last copy from hParams
     bool uploadPending = false;
+    bool paramsStale = false;  // tau changed on the host since the last upload
+    // Results the host reads, written by the kernels themselves into pinned
+    // coherent memory (no copy per value): hScalars by slot (ScalarSlot),
+    // hRows the trace rows [kTraceRows][p], hGram X^T X [p][p]; kScalars,
+    // kRows and kGram are the same memory's device addresses.
+    enum ScalarSlot { kSlotK = 0, kSlotQuad = 1, kSlotSum = 2, kSlotSumStar = 3, kSlotK0 = 4, kScalarSlots = 6 };
     double* hScalars = nullptr;
+    double* kScalars = nullptr;
+    double* hRows = nullptr;
+    double* kRows = nullptr;
+    double* hGram = nullptr;
+    double* kGram = nullptr;
     void* hX = nullptr;
-    double* dRows = nullptr;  // trace rows [kTraceRows][p]
-    double* dGram = nullptr;  // X^T X [p][p]
-    double* hRows = nullptr;  // pinned
-    double* hGram = nullptr;  // pinned
     TraceRows traceRows{};
     // gradient sources of a leapfrog step
     enum Mode { kExact = 0, kExactCollect = 1, kSurrogate = 2 };
@@ -105,6 +111,9 @@ struct Hmc {
     void copyLocationsToHost(std::vector<double>& out);
     void gatherTraceRows();
     void enqueueEnergies();
+    void launchEnergy(int slot, bool withSum);
+    void startIteration(int it);
+    void restoreStart();
     void enqueueAtKept(bool traitStep, bool recordIt, double tauStar);
     void gatherGram();
     void record(double* row, const double* rows, double logPost) const;

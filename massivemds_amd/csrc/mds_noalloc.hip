@@ -129,7 +129,7 @@ void evaluate_noalloc(Engine& e, const double* locations, double* sumOut, double
     } else {
         e.flush_row_write();
     }
-    e.launch_tile_pass(gradient, nullptr, tileSums, part);
+    e.launch_tile_pass(gradient, nullptr, tileSums, part, e.precision);
     e.launch_reduce_pass(gradient, tileSums, part, gradientOut, sumOut,
                          e.truncated ? 1.0 : 0.5 * e.precision);  // getSumOfIncrements (:198-202)
     ++e.fullEvaluations;
