@@ -782,12 +782,14 @@ struct Batch {
 
 struct ChainDesc {
     void* x;                // the entry's locations, Real [nb * 64][kdim]
+    void* xSave;            // their copy at the start of an iteration
+    int64_t xCount;         // nb * 64 * kdim
     double* mom;            // [n][p]
-    const double* grad;     // the entry's gradient [n][p]
+    double* grad;           // the entry's gradient [n][p]
+    double* gradSave;       // its copy
     const double* params;   // kHmcParamCount values
-    double* energy;         // [2]: kinetic energy, x^T T x sum
-    double* rows;           // [kTraceRows][p]
-    double* gram;           // [p][p]
+    double* rows;           // [kTraceRows][p], pinned host memory
+    double* gram;           // [p][p], pinned host memory
     TraceRows traceRows;
     uint64_t seed;
     int n, p, kdim;
@@ -795,10 +797,47 @@ struct ChainDesc {
 
 constexpr int kChainThreads = 256;
 
-__global__ __launch_bounds__(kChainThreads) void chains_momentum_kernel(const ChainDesc* __restrict__ cd,
-                                                                        uint64_t iteration) {
+// The start of an iteration for every chain in one launch: the momenta
+// (momentum_pair) and the copies of the locations and gradient that a
+// rejected move restores.
+template <typename Real>
+__global__ __launch_bounds__(kChainThreads) void chains_start_kernel(const ChainDesc* __restrict__ cd,
+                                                                     uint64_t iteration) {
     const ChainDesc& d = cd[blockIdx.y];
-    momentum_pair(d.mom, d.n * d.p, d.seed, iteration, blockIdx.x * kChainThreads + threadIdx.x);
+    const int64_t t = (int64_t)blockIdx.x * kChainThreads + threadIdx.x;
+    momentum_pair(d.mom, d.n * d.p, d.seed, iteration, (int)t);
+    if (t < d.xCount) ((Real*)d.xSave)[t] = ((const Real*)d.x)[t];
+    if (t < (int64_t)d.n * d.p) d.gradSave[t] = d.grad[t];
+}
+
+// Chains chain0 + y for the set bits y of mask (up to kMaskChains per launch):
+// what & 1 the locations, what & 2 the gradient, copied to their saves or
+// (restore) back from them.
+constexpr int kMaskChains = 256;
+struct ChainMask {
+    uint32_t w[kMaskChains / 32];
+};
+
+template <typename Real>
+__global__ __launch_bounds__(kChainThreads) void chains_copy_kernel(const ChainDesc* __restrict__ cd, ChainMask mask,
+                                                                    int chain0, int what, int restore) {
+    if (!((mask.w[blockIdx.y >> 5] >> (blockIdx.y & 31)) & 1u)) return;  // the same for the whole block
+    const ChainDesc& d = cd[chain0 + blockIdx.y];
+    const int64_t t = (int64_t)blockIdx.x * kChainThreads + threadIdx.x;
+    if ((what & 1) && t < d.xCount) {
+        Real* x = (Real*)d.x;
+        Real* xs = (Real*)d.xSave;
+        if (restore)
+            x[t] = xs[t];
+        else
+            xs[t] = x[t];
+    }
+    if ((what & 2) && t < (int64_t)d.n * d.p) {
+        if (restore)
+            d.grad[t] = d.gradSave[t];
+        else
+            d.gradSave[t] = d.grad[t];
+    }
 }
 
 template <typename Real, int FORM>
@@ -807,11 +846,17 @@ __global__ __launch_bounds__(kChainThreads) void chains_kick_kernel(const ChainD
     kick_row<Real, FORM>((Real*)d.x, d.mom, d.grad, d.params, d.n, d.p, d.kdim, blockIdx.x * kChainThreads + threadIdx.x);
 }
 
+// Chain c's energies into out[2 c], out[2 c + 1]; with sumsIn, its last
+// pass's sum into sumsOut[c] as well.
 template <typename Real>
-__global__ __launch_bounds__(kEnergyThreads) void chains_energy_kernel(const ChainDesc* __restrict__ cd) {
+__global__ __launch_bounds__(kEnergyThreads) void chains_energy_kernel(const ChainDesc* __restrict__ cd,
+                                                                       double* __restrict__ out,
+                                                                       const double* __restrict__ sumsIn,
+                                                                       double* __restrict__ sumsOut) {
     __shared__ double red[2][kEnergyThreads];
     const ChainDesc& d = cd[blockIdx.x];
-    energy_block<Real>((const Real*)d.x, d.mom, d.params, d.n, d.p, d.kdim, d.energy, red);
+    energy_block<Real>((const Real*)d.x, d.mom, d.params, d.n, d.p, d.kdim, out + 2 * blockIdx.x, red);
+    if (sumsIn && threadIdx.x == 0) sumsOut[blockIdx.x] = sumsIn[blockIdx.x];
 }
 
 template <typename Real>
@@ -851,18 +896,26 @@ struct BatchHmc {
     double* dGradSave = nullptr;
     char* dXSave = nullptr;
     double* dParams = nullptr;   // [C][kHmcParamCount]
-    double* dEnergy = nullptr;   // [C][2]
-    double* dRows = nullptr;     // [C][kTraceRows * kMaxDim]
-    double* dGram = nullptr;     // [C][kMaxDim * kMaxDim]
     ChainDesc* dChains = nullptr;
-    double* hParams = nullptr;   // pinned copies
+    double* hParams = nullptr;   // pinned, copied to dParams
+    // Written by the kernels themselves into pinned coherent memory and read
+    // after a stream wait (k*: the same memory's device addresses):
+    // energies [C][2] at the start and the end of a trajectory, each chain's
+    // last sum [C], trace rows [C][kTraceRows * kMaxDim], X^T X [C][kMaxDim^2].
     double* hEnergy0 = nullptr;
     double* hEnergy = nullptr;
     double* hSums = nullptr;
     double* hRows = nullptr;
     double* hGram = nullptr;
+    double* kEnergy0 = nullptr;
+    double* kEnergy = nullptr;
+    double* kSums = nullptr;
+    double* kRows = nullptr;
+    double* kGram = nullptr;
+    int64_t maxWork = 0;         // threads per chain of the start and copy kernels
     hipEvent_t uploadDone = nullptr;
     bool uploadPending = false;
+    bool paramsStale = false;    // a precision changed on the host since the last upload
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     BatchConfig savedCfg;
@@ -906,33 +959,34 @@ struct BatchHmc {
             check(hipMalloc((void**)&dGradSave, momCount * sizeof(double)), "hipMalloc(gradient copies)");
             check(hipMalloc((void**)&dXSave, xSaveBytes), "hipMalloc(location copies)");
             check(hipMalloc((void**)&dParams, (size_t)C * kHmcParamCount * sizeof(double)), "hipMalloc(parameters)");
-            check(hipMalloc((void**)&dEnergy, (size_t)C * 2 * sizeof(double)), "hipMalloc(energies)");
-            check(hipMalloc((void**)&dRows, (size_t)C * kTraceRows * kMaxDim * sizeof(double)), "hipMalloc(rows)");
-            check(hipMalloc((void**)&dGram, (size_t)C * kMaxDim * kMaxDim * sizeof(double)), "hipMalloc(gram)");
             check(hipMalloc((void**)&dChains, (size_t)C * sizeof(ChainDesc)), "hipMalloc(chains)");
             check(hipHostMalloc((void**)&hParams, (size_t)C * kHmcParamCount * sizeof(double), hipHostMallocDefault),
                   "hipHostMalloc");
-            check(hipHostMalloc((void**)&hEnergy0, (size_t)C * 2 * sizeof(double), hipHostMallocDefault),
-                  "hipHostMalloc");
-            check(hipHostMalloc((void**)&hEnergy, (size_t)C * 2 * sizeof(double), hipHostMallocDefault),
-                  "hipHostMalloc");
-            check(hipHostMalloc((void**)&hSums, (size_t)C * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-            check(hipHostMalloc((void**)&hRows, (size_t)C * kTraceRows * kMaxDim * sizeof(double),
-                                hipHostMallocDefault), "hipHostMalloc");
-            check(hipHostMalloc((void**)&hGram, (size_t)C * kMaxDim * kMaxDim * sizeof(double), hipHostMallocDefault),
-                  "hipHostMalloc");
+            const auto pinned = [](double** h, double** k, size_t count) {
+                check(hipHostMalloc((void**)h, count * sizeof(double), hipHostMallocCoherent), "hipHostMalloc");
+                check(hipHostGetDevicePointer((void**)k, *h, 0), "hipHostGetDevicePointer");
+            };
+            pinned(&hEnergy0, &kEnergy0, (size_t)C * 2);
+            pinned(&hEnergy, &kEnergy, (size_t)C * 2);
+            pinned(&hSums, &kSums, (size_t)C);
+            pinned(&hRows, &kRows, (size_t)C * kTraceRows * kMaxDim);
+            pinned(&hGram, &kGram, (size_t)C * kMaxDim * kMaxDim);
             check(hipEventCreateWithFlags(&uploadDone, hipEventDisableTiming), "hipEventCreate");
             std::memset(hParams, 0, (size_t)C * kHmcParamCount * sizeof(double));
             std::vector<ChainDesc> desc(C);
             for (int c = 0; c < C; ++c) {
                 ChainDesc& d = desc[c];
                 d.x = b.e[c].dX;
+                d.xSave = dXSave + ch[c].xSaveOffset;
+                d.xCount = (int64_t)(ch[c].xBytes / b.realSize);
                 d.mom = dMom + ch[c].momOffset;
                 d.grad = b.e[c].dGrad;
+                d.gradSave = dGradSave + ch[c].momOffset;
                 d.params = dParams + (size_t)c * kHmcParamCount;
-                d.energy = dEnergy + 2 * c;
-                d.rows = dRows + (size_t)c * kTraceRows * kMaxDim;
-                d.gram = dGram + (size_t)c * kMaxDim * kMaxDim;
+                d.rows = kRows + (size_t)c * kTraceRows * kMaxDim;
+                d.gram = kGram + (size_t)c * kMaxDim * kMaxDim;
+                // the momentum pairs are fewer than the n * p gradient values
+                maxWork = std::max(maxWork, std::max<int64_t>(d.xCount, (int64_t)ch[c].n * ch[c].p));
                 d.traceRows = trace_rows(ch[c].n);
                 d.seed = opt.seed + (uint64_t)c;
                 d.n = ch[c].n;
@@ -974,9 +1028,6 @@ struct BatchHmc {
         (void)hipFree(dGradSave);
         (void)hipFree(dXSave);
         (void)hipFree(dParams);
-        (void)hipFree(dEnergy);
-        (void)hipFree(dRows);
-        (void)hipFree(dGram);
         (void)hipFree(dChains);
         (void)hipHostFree(hParams);
         (void)hipHostFree(hEnergy0);
@@ -985,10 +1036,11 @@ struct BatchHmc {
         (void)hipHostFree(hRows);
         (void)hipHostFree(hGram);
         if (uploadDone) (void)hipEventDestroy(uploadDone);
-        dMom = dGradSave = dParams = dEnergy = dRows = dGram = nullptr;
+        dMom = dGradSave = dParams = nullptr;
         dXSave = nullptr;
         dChains = nullptr;
         hParams = hEnergy0 = hEnergy = hSums = hRows = hGram = nullptr;
+        kEnergy0 = kEnergy = kSums = kRows = kGram = nullptr;
         uploadDone = nullptr;
         // the batch evaluates with the host precisions again
         for (BatchEntry& en : b.e) {
@@ -1003,6 +1055,7 @@ struct BatchHmc {
     // The pinned block is rewritten only after the previous upload from it has run.
     void uploadParams() {
         if (uploadPending) check(hipEventSynchronize(uploadDone), "hipEventSynchronize(parameters)");
+        paramsStale = false;
         for (int c = 0; c < C; ++c) {
             double* h = hParams + (size_t)c * kHmcParamCount;
             h[kHmcTau] = ch[c].tau;
@@ -1026,12 +1079,53 @@ struct BatchHmc {
         check(hipGetLastError(), "chains kick kernel");
     }
 
-    void energies() {
+    // Every chain's energies into out (kEnergy0 or kEnergy); withSums: the
+    // last pass's sums into kSums as well.
+    void energies(double* out, bool withSums) {
+        const double* in = withSums ? b.dSums : nullptr;
         if (b.fp32)
-            hipLaunchKernelGGL(chains_energy_kernel<float>, dim3(C), dim3(kEnergyThreads), 0, b.stream, dChains);
+            hipLaunchKernelGGL(chains_energy_kernel<float>, dim3(C), dim3(kEnergyThreads), 0, b.stream, dChains, out,
+                               in, kSums);
         else
-            hipLaunchKernelGGL(chains_energy_kernel<double>, dim3(C), dim3(kEnergyThreads), 0, b.stream, dChains);
+            hipLaunchKernelGGL(chains_energy_kernel<double>, dim3(C), dim3(kEnergyThreads), 0, b.stream, dChains, out,
+                               in, kSums);
         check(hipGetLastError(), "chains energy kernel");
+    }
+
+    void startChains(int it) {
+        const dim3 grid((unsigned)((maxWork + kChainThreads - 1) / kChainThreads), C);
+        if (b.fp32)
+            hipLaunchKernelGGL(chains_start_kernel<float>, grid, dim3(kChainThreads), 0, b.stream, dChains,
+                               (uint64_t)it);
+        else
+            hipLaunchKernelGGL(chains_start_kernel<double>, grid, dim3(kChainThreads), 0, b.stream, dChains,
+                               (uint64_t)it);
+        check(hipGetLastError(), "chains start kernel");
+    }
+
+    // Copies of the selected chains (what: 1 locations, 2 gradient, 3
+    // both) to their saves, or back from them; one launch per kMaskChains
+    // chains that has a selected one.
+    void copyChains(const std::vector<char>& sel, int what, bool restore) {
+        for (int c0 = 0; c0 < C; c0 += kMaskChains) {
+            const int count = std::min(kMaskChains, C - c0);
+            ChainMask mask{};
+            bool any = false;
+            for (int y = 0; y < count; ++y)
+                if (sel[c0 + y]) {
+                    mask.w[y >> 5] |= 1u << (y & 31);
+                    any = true;
+                }
+            if (!any) continue;
+            const dim3 grid((unsigned)((maxWork + kChainThreads - 1) / kChainThreads), count);
+            if (b.fp32)
+                hipLaunchKernelGGL(chains_copy_kernel<float>, grid, dim3(kChainThreads), 0, b.stream, dChains, mask,
+                                   c0, what, restore ? 1 : 0);
+            else
+                hipLaunchKernelGGL(chains_copy_kernel<double>, grid, dim3(kChainThreads), 0, b.stream, dChains, mask,
+                                   c0, what, restore ? 1 : 0);
+            check(hipGetLastError(), "chains copy kernel");
+        }
     }
 
     // Every chain's likelihood and gradient at its current locations.
@@ -1070,17 +1164,6 @@ struct BatchHmc {
               "hipMemcpy(sums)");
     }
 
-    void saveGradient(int c) {
-        const size_t bytes = (size_t)ch[c].n * ch[c].p * sizeof(double);
-        check(hipMemcpyAsync(dGradSave + ch[c].momOffset, b.e[c].dGrad, bytes, hipMemcpyDeviceToDevice, b.stream),
-              "hipMemcpy(gradient copy)");
-    }
-
-    void restoreGradient(int c) {
-        const size_t bytes = (size_t)ch[c].n * ch[c].p * sizeof(double);
-        check(hipMemcpyAsync(b.e[c].dGrad, dGradSave + ch[c].momOffset, bytes, hipMemcpyDeviceToDevice, b.stream),
-              "hipMemcpy(gradient restore)");
-    }
 
     void run(const double* x0, double* traces, double* finalX, mds_hip_hmc_stats* stats) {
         using clock = std::chrono::steady_clock;
@@ -1092,10 +1175,7 @@ struct BatchHmc {
         size_t momTotal = 0;
         for (const Chain& k : ch) momTotal += (size_t)k.n * k.p;
         check(hipMemsetAsync(dMom, 0, momTotal * sizeof(double), b.stream), "hipMemset(momentum)");
-        energies();
-        check(hipMemcpyAsync(hEnergy, dEnergy, (size_t)C * 2 * sizeof(double), hipMemcpyDeviceToHost, b.stream),
-              "hipMemcpy(energies)");
-        readSums();
+        energies(kEnergy, true);
         check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
         for (int c = 0; c < C; ++c) {
             Chain& k = ch[c];
@@ -1106,9 +1186,7 @@ struct BatchHmc {
         if (opt.useGraph) buildGraph();
         const double daGamma = 0.05, daT0 = 10.0, daKappa = 0.75;
         auto tSampling = clock::now();
-        int maxPairs = 0;
-        for (const Chain& k : ch) maxPairs = std::max(maxPairs, (k.n * k.p + 1) / 2);
-        const dim3 momGrid((maxPairs + kChainThreads - 1) / kChainThreads, C);
+        std::vector<char> sel(C);
         for (int it = 0; it < opt.iterations; ++it) {
             if (it == opt.burnIn) {
                 tSampling = clock::now();
@@ -1117,28 +1195,18 @@ struct BatchHmc {
                     uploadParams();
                 }
             }
-            hipLaunchKernelGGL(chains_momentum_kernel, momGrid, dim3(kChainThreads), 0, b.stream, dChains,
-                               (uint64_t)it);
-            check(hipGetLastError(), "chains momentum kernel");
-            energies();
-            check(hipMemcpyAsync(hEnergy0, dEnergy, (size_t)C * 2 * sizeof(double), hipMemcpyDeviceToHost, b.stream),
-                  "hipMemcpy(energies)");
-            for (int c = 0; c < C; ++c) {
-                check(hipMemcpyAsync(dXSave + ch[c].xSaveOffset, b.e[c].dX, ch[c].xBytes, hipMemcpyDeviceToDevice,
-                                     b.stream),
-                      "hipMemcpy(location copy)");
-                saveGradient(c);
-            }
+            // the momenta and the copies a rejection restores, then the
+            // starting kinetic energies (read after the trajectory's wait)
+            startChains(it);
+            energies(kEnergy0, false);
             if (graphExec)
                 check(hipGraphLaunch(graphExec, b.stream), "hipGraphLaunch");
             else
                 trajectory();
-            energies();
-            check(hipMemcpyAsync(hEnergy, dEnergy, (size_t)C * 2 * sizeof(double), hipMemcpyDeviceToHost, b.stream),
-                  "hipMemcpy(energies)");
-            readSums();
+            energies(kEnergy, true);
             check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
             bool adapted = false;
+            std::fill(sel.begin(), sel.end(), 0);
             for (int c = 0; c < C; ++c) {
                 Chain& k = ch[c];
                 const double K0 = hEnergy0[2 * c], K1 = hEnergy[2 * c];
@@ -1154,10 +1222,7 @@ struct BatchHmc {
                     k.quad = quad1;
                     ++k.accepted;
                 } else {
-                    check(hipMemcpyAsync(b.e[c].dX, dXSave + k.xSaveOffset, k.xBytes, hipMemcpyDeviceToDevice,
-                                         b.stream),
-                          "hipMemcpy(location restore)");
-                    restoreGradient(c);
+                    sel[c] = 1;  // locations and gradient restored below
                 }
                 k.evaluations += opt.leapfrogSteps;
                 if (opt.adaptStepSize && it < opt.burnIn) {
@@ -1172,10 +1237,12 @@ struct BatchHmc {
                     adapted = true;
                 }
             }
+            copyChains(sel, 3, true);  // the rejected chains' locations and gradients
             if (adapted) uploadParams();
 
             // trace rows and X^T X of the kept locations (the precision step
-            // does not move them), read after the next wait
+            // does not move them), written into pinned memory and read after
+            // the next wait
             const bool traitStep = opt.learnTraitPrecision && it < opt.iterations - 1 &&
                                    ((it + 1) % opt.traitInvWeight == 0);
             const bool recordIt = it >= opt.burnIn && traces;
@@ -1187,9 +1254,6 @@ struct BatchHmc {
                     hipLaunchKernelGGL(chains_gram_kernel<double>, dim3(maxP * maxP, C), dim3(kGramThreads), 0,
                                        b.stream, dChains);
                 check(hipGetLastError(), "chains gram kernel");
-                check(hipMemcpyAsync(hGram, dGram, (size_t)C * kMaxDim * kMaxDim * sizeof(double),
-                                     hipMemcpyDeviceToHost, b.stream),
-                      "hipMemcpy(gram)");
             }
             if (recordIt) {
                 if (b.fp32)
@@ -1199,9 +1263,6 @@ struct BatchHmc {
                     hipLaunchKernelGGL(chains_rows_kernel<double>, dim3(C), dim3(kTraceRows * kMaxDim), 0, b.stream,
                                        dChains);
                 check(hipGetLastError(), "chains rows kernel");
-                check(hipMemcpyAsync(hRows, dRows, (size_t)C * kTraceRows * kMaxDim * sizeof(double),
-                                     hipMemcpyDeviceToHost, b.stream),
-                      "hipMemcpy(rows)");
             }
 
             if (opt.learnPrecision) {
@@ -1214,19 +1275,21 @@ struct BatchHmc {
                         const double tauStar = std::fabs(lo + (hi - lo) * k.rng.uniform());
                         ++k.proposedTau;
                         k.proposing = tauStar > 0.0;
+                        sel[c] = k.proposing ? 1 : 0;
                         if (k.proposing) {
                             k.savedTau = k.tau;
                             k.tau = tauStar;
-                            saveGradient(c);
                             any = true;
                         }
                     }
                     if (any) {
+                        copyChains(sel, 2, false);  // the proposing chains' gradients
                         uploadParams();
                         evaluateAll();
                         readSums();
                         check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
                         bool restored = false;
+                        std::fill(sel.begin(), sel.end(), 0);
                         for (int c = 0; c < C; ++c) {
                             Chain& k = ch[c];
                             if (!k.proposing) continue;
@@ -1243,11 +1306,16 @@ struct BatchHmc {
                                 ++k.evaluations;
                             } else {
                                 k.tau = k.savedTau;
-                                restoreGradient(c);
+                                sel[c] = 1;
                                 restored = true;
                             }
                         }
-                        if (restored) uploadParams();
+                        // the rejected chains' gradients back; their precisions
+                        // go with the iteration's last upload
+                        if (restored) {
+                            copyChains(sel, 2, true);
+                            paramsStale = true;
+                        }
                     }
                 } else {
                     // Gibbs: tau | X ~ Gamma(1 + N(N-1)/4, C / 2) with C the unscaled
@@ -1290,6 +1358,7 @@ struct BatchHmc {
                 }
                 uploadParams();
             }
+            if (paramsStale) uploadParams();
             if (recordIt)
                 for (int c = 0; c < C; ++c) {
                     const Chain& k = ch[c];
