@@ -211,29 +211,6 @@ Hmc::Hmc(Engine& engine, const mds_hip_hmc_options& options) : e(engine), opt(op
     if (opt.traitInvWeight < 1) throw Error(MDS_HIP_ERR_ARGUMENT, "traitInvWeight must be >= 1");
     n = e.N;
     p = e.P;
-    const size_t np = (size_t)n * p;
-    const size_t xBytes = (size_t)e.npad * e.kdim * e.realSize;
-    check(hipMalloc((void**)&dMom, np * sizeof(double)), "hipMalloc(momentum)");
-    check(hipMalloc((void**)&dGradSave, np * sizeof(double)), "hipMalloc(gradient copy)");
-    check(hipMalloc(&dXSave, xBytes), "hipMalloc(location copy)");
-    check(hipMalloc((void**)&dParams, kHmcParamCount * sizeof(double)), "hipMalloc(parameters)");
-    check(hipHostMalloc((void**)&hParams, kHmcParamCount * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-    check(hipHostMalloc(&hX, xBytes, hipHostMallocDefault), "hipHostMalloc");
-    // written by kernels (coherent pinned memory, read after a stream wait)
-    check(hipHostMalloc((void**)&hScalars, kScalarSlots * sizeof(double), hipHostMallocCoherent), "hipHostMalloc");
-    check(hipHostMalloc((void**)&hRows, kTraceRows * kMaxDim * sizeof(double), hipHostMallocCoherent),
-          "hipHostMalloc");
-    check(hipHostMalloc((void**)&hGram, kMaxDim * kMaxDim * sizeof(double), hipHostMallocCoherent), "hipHostMalloc");
-    check(hipHostGetDevicePointer((void**)&kScalars, hScalars, 0), "hipHostGetDevicePointer");
-    check(hipHostGetDevicePointer((void**)&kRows, hRows, 0), "hipHostGetDevicePointer");
-    check(hipHostGetDevicePointer((void**)&kGram, hGram, 0), "hipHostGetDevicePointer");
-    check(hipEventCreateWithFlags(&uploadDone, hipEventDisableTiming), "hipEventCreate");
-    traceRows = trace_rows(n);
-    std::memset(hParams, 0, kHmcParamCount * sizeof(double));
-    tau = e.precision;
-    trait.assign((size_t)p * p, 0.0);
-    for (int d = 0; d < p; ++d) trait[d * p + d] = 1.0;  // solve(traitVcv) with traitVcv = I_P
-    eps = opt.stepSize > 0.0 ? opt.stepSize : opt.trajectory / opt.leapfrogSteps;
     if (opt.useSurrogate || opt.collectSnapshots) {
         if (!opt.surrogate || !opt.surrogate->s)
             throw Error(MDS_HIP_ERR_ARGUMENT, "surrogate handle required for useSurrogate / collectSnapshots");
@@ -242,13 +219,49 @@ Hmc::Hmc(Engine& engine, const mds_hip_hmc_options& options) : e(engine), opt(op
             throw Error(MDS_HIP_ERR_ARGUMENT, "surrogate built for another N or P");
         if (sur->mlp.device != e.device) throw Error(MDS_HIP_ERR_ARGUMENT, "surrogate on another device");
     }
+    const size_t np = (size_t)n * p;
+    const size_t xBytes = (size_t)e.npad * e.kdim * e.realSize;
+    try {
+        check(hipMalloc((void**)&dMom, np * sizeof(double)), "hipMalloc(momentum)");
+        check(hipMalloc((void**)&dGradSave, np * sizeof(double)), "hipMalloc(gradient copy)");
+        check(hipMalloc(&dXSave, xBytes), "hipMalloc(location copy)");
+        check(hipMalloc((void**)&dParams, kHmcParamCount * sizeof(double)), "hipMalloc(parameters)");
+        check(hipHostMalloc((void**)&hParams, kHmcParamCount * sizeof(double), hipHostMallocDefault),
+              "hipHostMalloc");
+        check(hipHostMalloc(&hX, xBytes, hipHostMallocDefault), "hipHostMalloc");
+        // written by kernels (coherent pinned memory, read after a stream wait)
+        check(hipHostMalloc((void**)&hScalars, kScalarSlots * sizeof(double), hipHostMallocCoherent),
+              "hipHostMalloc");
+        check(hipHostMalloc((void**)&hRows, kTraceRows * kMaxDim * sizeof(double), hipHostMallocCoherent),
+              "hipHostMalloc");
+        check(hipHostMalloc((void**)&hGram, kMaxDim * kMaxDim * sizeof(double), hipHostMallocCoherent),
+              "hipHostMalloc");
+        check(hipHostGetDevicePointer((void**)&kScalars, hScalars, 0), "hipHostGetDevicePointer");
+        check(hipHostGetDevicePointer((void**)&kRows, hRows, 0), "hipHostGetDevicePointer");
+        check(hipHostGetDevicePointer((void**)&kGram, hGram, 0), "hipHostGetDevicePointer");
+        check(hipEventCreateWithFlags(&uploadDone, hipEventDisableTiming), "hipEventCreate");
+    } catch (...) {
+        release();
+        throw;
+    }
+    traceRows = trace_rows(n);
+    std::memset(hParams, 0, kHmcParamCount * sizeof(double));
+    tau = e.precision;
+    trait.assign((size_t)p * p, 0.0);
+    for (int d = 0; d < p; ++d) trait[d * p + d] = 1.0;  // solve(traitVcv) with traitVcv = I_P
+    eps = opt.stepSize > 0.0 ? opt.stepSize : opt.trajectory / opt.leapfrogSteps;
 }
 
-Hmc::~Hmc() {
+Hmc::~Hmc() { release(); }
+
+// Frees what the constructor allocated (also after a failure part way).
+void Hmc::release() {
     (void)hipStreamSynchronize(e.stream);
     for (int m = 0; m < 3; ++m) {
         if (graphExec[m]) (void)hipGraphExecDestroy(graphExec[m]);
         if (graph[m]) (void)hipGraphDestroy(graph[m]);
+        graphExec[m] = nullptr;
+        graph[m] = nullptr;
     }
     (void)hipFree(dMom);
     (void)hipFree(dGradSave);
@@ -260,6 +273,10 @@ Hmc::~Hmc() {
     (void)hipHostFree(hRows);
     (void)hipHostFree(hGram);
     if (uploadDone) (void)hipEventDestroy(uploadDone);
+    dMom = dGradSave = dParams = nullptr;
+    dXSave = hX = nullptr;
+    hParams = hScalars = hRows = hGram = kScalars = kRows = kGram = nullptr;
+    uploadDone = nullptr;
 }
 
 // The pinned block is rewritten only after the previous upload from it has

@@ -98,6 +98,7 @@ struct Hmc {
 
     Hmc(Engine& engine, const mds_hip_hmc_options& options);
     ~Hmc();
+    void release();
     void uploadParams();
     template <int FORM>
     void kick();
