@@ -46,6 +46,14 @@ struct ReduceKick {
     const double* params = nullptr;
 };
 
+// Where one pass (Engine::evaluate) sends its results besides dGrad and dSum.
+struct PassOut {
+    double tau = 0.0;             // > 0: at this precision instead of the engine's (no devTau)
+    double* sum = nullptr;        // the sum here instead of dSum, which keeps its value
+    double* sumHost = nullptr;    // a second copy of the sum (pinned host memory)
+    double* gradHost = nullptr;   // a second copy of the gradient rows (pinned host memory)
+};
+
 struct Engine {
     int P = 0, N = 0, kdim = 0, device = 0;
     long flags = 0;
@@ -70,8 +78,10 @@ struct Engine {
     void* dPart = nullptr;     // gradient partials in the engine's real type
     double* dGrad = nullptr;
     double* dSum = nullptr;    // dGrad + N * P: the sum follows the gradient, so one copy reads both
-    double* hGrad = nullptr;   // pinned
-    double* hSum = nullptr;    // pinned
+    double* hGrad = nullptr;   // pinned, coherent (the reduce kernel can write it directly)
+    double* kGrad = nullptr;   // its device address
+    double* hSum = nullptr;    // pinned, coherent: the reduce kernel can write it directly
+    double* kSum = nullptr;    // hSum's device address
     void* hXBuf[2] = {nullptr, nullptr};  // pinned staging for location uploads, used in turn
     hipEvent_t hXDone[2] = {nullptr, nullptr};  // the last upload from each has run
     bool hXBusy[2] = {false, false};
@@ -108,14 +118,20 @@ struct Engine {
     // Copy on write: storeState only marks the current buffer as shared with
     // the stored state; the next write of a gradient moves to the other
     // buffer first.
-    double* hGradStored = nullptr;  // pinned
+    double* hGradStored = nullptr;  // pinned, coherent
+    double* kGradStored = nullptr;  // its device address
     bool storedGradFresh = false;
     bool storedSharesCurrent = false;  // the stored gradient is the one in hGrad
+
+    void swap_host_gradients() {
+        std::swap(hGrad, hGradStored);
+        std::swap(kGrad, kGradStored);
+    }
 
     // hGrad is about to be overwritten: keep a stored gradient that shares it
     void own_host_gradient() {
         if (storedSharesCurrent) {
-            std::swap(hGrad, hGradStored);
+            swap_host_gradients();
             storedSharesCurrent = false;
         }
     }
@@ -188,11 +204,15 @@ struct Engine {
             check(hipMalloc((void**)&dGrad, ((size_t)N * P + 1) * sizeof(double)), "hipMalloc(gradient)");
             dSum = dGrad + (size_t)N * P;
             // the host copies hold the sum after the gradient as well
-            check(hipHostMalloc((void**)&hGrad, ((size_t)N * P + 1) * sizeof(double), hipHostMallocDefault),
+            check(hipHostMalloc((void**)&hGrad, ((size_t)N * P + 1) * sizeof(double), hipHostMallocCoherent),
                   "hipHostMalloc(gradient)");
-            check(hipHostMalloc((void**)&hGradStored, ((size_t)N * P + 1) * sizeof(double), hipHostMallocDefault),
+            check(hipHostMalloc((void**)&hGradStored, ((size_t)N * P + 1) * sizeof(double), hipHostMallocCoherent),
                   "hipHostMalloc(stored gradient)");
-            check(hipHostMalloc((void**)&hSum, sizeof(double), hipHostMallocDefault), "hipHostMalloc(sum)");
+            check(hipHostGetDevicePointer((void**)&kGrad, hGrad, 0), "hipHostGetDevicePointer(gradient)");
+            check(hipHostGetDevicePointer((void**)&kGradStored, hGradStored, 0),
+                  "hipHostGetDevicePointer(stored gradient)");
+            check(hipHostMalloc((void**)&hSum, sizeof(double), hipHostMallocCoherent), "hipHostMalloc(sum)");
+            check(hipHostGetDevicePointer((void**)&kSum, hSum, 0), "hipHostGetDevicePointer(sum)");
             for (int k = 0; k < 2; ++k) {
                 check(hipHostMalloc(&hXBuf[k], (size_t)npad * kdim * realSize, hipHostMallocDefault),
                       "hipHostMalloc(locations)");
@@ -244,7 +264,7 @@ struct Engine {
         hRowPartial = nullptr;
         dDelta = dX[0] = dX[1] = nullptr;
         dTiles = nullptr;
-        dTileSums = dGrad = dSum = hGrad = hGradStored = hSum = nullptr;
+        dTileSums = dGrad = dSum = hGrad = hGradStored = hSum = kSum = kGrad = kGradStored = nullptr;
         dPart = nullptr;
         stream = nullptr;
     }
@@ -475,7 +495,7 @@ struct Engine {
         storePending = true;
         invalidate_device_results();
         if (storedGradFresh) {
-            if (!storedSharesCurrent) std::swap(hGrad, hGradStored);
+            if (!storedSharesCurrent) swap_host_gradients();
             hostGradFresh = true;
         }
         storedGradFresh = false;
@@ -522,7 +542,8 @@ struct Engine {
     // each row then moves its momentum (and location) at the current
     // locations.
     void launch_reduce_pass(bool gradient, const double* tileSums, const void* part, double* grad, double* sumOut,
-                            double sumScale, const ReduceKick& kick = ReduceKick()) {
+                            double sumScale, const ReduceKick& kick = ReduceKick(), double* sumHost = nullptr,
+                            double* gradHost = nullptr) {
         ReduceLaunch r;
         r.part = part;
         r.fp32 = fp32;
@@ -534,7 +555,9 @@ struct Engine {
         r.p = P;
         r.gradient = gradient;
         r.grad = grad;
+        r.gradHost = gradHost;
         r.sumOut = sumOut;
+        r.sumHost = sumHost;
         r.sumScale = sumScale;
         r.kickForm = kick.form;
         r.kickX = dX[cur];
@@ -543,12 +566,9 @@ struct Engine {
         check(launch_reduce(r, stream), "reduce kernel");
     }
 
-    // One fused pass at the current locations.  tauValue > 0: at that
-    // precision instead of the engine's (when devTau is null); sumOut: the
-    // sum goes there (a device-visible address, pinned host memory included)
-    // instead of dSum, which then keeps its previous value.
+    // One fused pass at the current locations (out: see PassOut).
     void evaluate(bool gradient, const double* devTau = nullptr, const ReduceKick& kick = ReduceKick(),
-                  double tauValue = 0.0, double* sumOut = nullptr) {
+                  const PassOut& out = PassOut()) {
         if (!dataSet) throw Error(MDS_HIP_ERR_STATE, "pairwise data not set");
         if (!precisionSet) throw Error(MDS_HIP_ERR_STATE, "precision not set");
         flush_row_write();
@@ -559,18 +579,19 @@ struct Engine {
             e2 = take_event();
             check(hipEventRecord(e0, stream), "hipEventRecord");
         }
-        launch_tile_pass(gradient, devTau, dTileSums, dPart, tauValue > 0.0 ? tauValue : precision);
+        launch_tile_pass(gradient, devTau, dTileSums, dPart, out.tau > 0.0 ? out.tau : precision);
         if (timing) check(hipEventRecord(e1, stream), "hipEventRecord");
-        launch_reduce_pass(gradient, dTileSums, dPart, dGrad, sumOut ? sumOut : dSum, 1.0, kick);
+        launch_reduce_pass(gradient, dTileSums, dPart, dGrad, out.sum ? out.sum : dSum, 1.0, kick, out.sumHost,
+                           out.gradHost);
         if (timing) {
             check(hipEventRecord(e2, stream), "hipEventRecord");
             pending.push_back(e0);
             pending.push_back(e1);
             pending.push_back(e2);
         }
-        const bool ownTau = tauValue <= 0.0;
-        if (!sumOut) devSumFresh = ownTau;  // dSum rewritten
-        if (gradient) gradKnown = ownTau;   // dGrad rewritten
+        const bool ownTau = out.tau <= 0.0;
+        if (!out.sum) devSumFresh = ownTau;  // dSum rewritten
+        if (gradient) gradKnown = ownTau;    // dGrad rewritten
         hostGradFresh = false;
         ++fullEvaluations;
     }
@@ -582,27 +603,32 @@ struct Engine {
                 applyRowUpdate();
                 pendingRow = -1;
             } else {
-                bool gradCopied = false;
+                bool gradOnHost = false, sumOnHost = false;
                 if (!devSumFresh) {
                     const bool predictGradient = gradRequestRate > kPredictRate;
-                    evaluate(predictGradient);
-                    stateEval = predictGradient ? StateEval::Fused : StateEval::SumOnly;
+                    // the reduce kernel writes the sum (into hSum), or with a
+                    // predicted gradient the gradient and the sum after it
+                    // (into hGrad), itself: no copy
+                    PassOut out;
                     if (predictGradient) {
-                        // the gradient and the sum stored after it, in one copy
                         own_host_gradient();
-                        check(hipMemcpyAsync(hGrad, dGrad, ((size_t)N * P + 1) * sizeof(double),
-                                             hipMemcpyDeviceToHost, stream),
-                              "hipMemcpy(gradient)");
-                        gradCopied = true;
+                        out.gradHost = kGrad;
+                        out.sumHost = kGrad + (size_t)N * P;
+                        gradOnHost = true;
+                    } else {
+                        out.sumHost = kSum;
+                        sumOnHost = true;
                     }
+                    evaluate(predictGradient, nullptr, ReduceKick(), out);
+                    stateEval = predictGradient ? StateEval::Fused : StateEval::SumOnly;
                 }
-                if (!gradCopied)
+                if (!gradOnHost && !sumOnHost)
                     check(hipMemcpyAsync(hSum, dSum, sizeof(double), hipMemcpyDeviceToHost, stream),
                           "hipMemcpy(sum)");
                 check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-                sumOfIncrements = gradCopied ? hGrad[(size_t)N * P] : *hSum;
+                sumOfIncrements = gradOnHost ? hGrad[(size_t)N * P] : *hSum;
                 sumFromFullEvaluation();
-                if (gradCopied) hostGradFresh = true;
+                if (gradOnHost) hostGradFresh = true;
             }
             sumKnown = true;
         }
@@ -618,16 +644,22 @@ struct Engine {
             std::memcpy(result, hGrad, length * sizeof(double));
             return;
         }
-        if (!gradKnown) {
-            evaluate(true);
-            if (stateEval == StateEval::None) stateEval = StateEval::Fused;
-        }
         const bool fillSum = !sumKnown;
         own_host_gradient();
-        // the gradient, and the sum stored after it when it is wanted too
-        check(hipMemcpyAsync(hGrad, dGrad, (length + (fillSum ? 1 : 0)) * sizeof(double), hipMemcpyDeviceToHost,
-                             stream),
-              "hipMemcpy(gradient)");
+        // the gradient, and the sum stored after it when it is wanted too:
+        // written into hGrad by the pass itself, or copied from the device
+        // when the pass ran before
+        if (!gradKnown) {
+            PassOut out;
+            out.gradHost = kGrad;
+            if (fillSum) out.sumHost = kGrad + length;
+            evaluate(true, nullptr, ReduceKick(), out);
+            if (stateEval == StateEval::None) stateEval = StateEval::Fused;
+        } else {
+            check(hipMemcpyAsync(hGrad, dGrad, (length + (fillSum ? 1 : 0)) * sizeof(double),
+                                 hipMemcpyDeviceToHost, stream),
+                  "hipMemcpy(gradient)");
+        }
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         std::memcpy(result, hGrad, length * sizeof(double));
         hostGradFresh = true;

@@ -410,7 +410,12 @@ void Hmc::launchEnergy(int slot, bool withSum) {
 void Hmc::enqueueAtKept(bool traitStep, bool recordIt, double tauStar) {
     if (traitStep) gatherGram();
     if (recordIt) gatherTraceRows();
-    if (tauStar > 0.0) e.evaluate(false, nullptr, ReduceKick(), tauStar, kScalars + kSlotSumStar);
+    if (tauStar > 0.0) {
+        PassOut out;
+        out.tau = tauStar;
+        out.sum = kScalars + kSlotSumStar;
+        e.evaluate(false, nullptr, ReduceKick(), out);
+    }
 }
 
 double mds_log_likelihood(double rawSum, double t, int n, bool truncated) {
@@ -693,7 +698,9 @@ void Hmc::run(const double* x0, double* trace, double* finalX, mds_hip_hmc_stats
                     ++acceptedTau;
                     // the gradient at the new tau, passed by value; the device
                     // parameter block follows at the end of the iteration
-                    e.evaluate(true, nullptr, ReduceKick(), tau);
+                    PassOut out;
+                    out.tau = tau;
+                    e.evaluate(true, nullptr, ReduceKick(), out);
                     paramsStale = true;
                     ++evaluations;
                 }

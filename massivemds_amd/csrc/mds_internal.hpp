@@ -53,7 +53,9 @@ struct ReduceLaunch {
     int p;                   // output dimension (<= kdim)
     bool gradient;
     double* grad;            // [n][p]
+    double* gradHost = nullptr;  // optional second copy of it (pinned host memory)
     double* sumOut;          // 1 double: (0.5 * sum of tile sums) * sumScale
+    double* sumHost = nullptr;  // optional second copy of it (pinned host memory)
     double sumScale = 1.0;
     // Optional HMC kick of each gradient row right after it is written
     // (kickForm: a KickForm of the sampler, or -1 for none): the locations

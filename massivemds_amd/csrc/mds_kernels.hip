@@ -68,7 +68,10 @@ constexpr int kReduceThreads = kReduceWaves * kWave;
 // column blocks J (J = w mod 8), combined through LDS in fixed order.
 // Last block: the likelihood, a fixed-order FP64 sum of the tile partials,
 // halved as in src/NewMultiDimensionalScaling.hpp:556, then scaled by
-// sumScale (1, or 0.5 tau for the untruncated value of :198-202).
+// sumScale (1, or 0.5 tau for the untruncated value of :198-202), into
+// sumOut and, when sumHost is set, also there (pinned host memory the caller
+// reads after a stream wait, with no copy command); likewise the gradient
+// rows into grad and gradHost.
 // KICK >= 0: each row's thread then applies the sampler's kick of that form
 // to its row (kick_row, reading back the gradient it has just written), so
 // that a leapfrog step needs no kick launch of its own; the reduce blocks
@@ -76,7 +79,8 @@ constexpr int kReduceThreads = kReduceWaves * kWave;
 template <typename PartT, int P, int KICK>
 __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
     const PartT* __restrict__ part, const double* __restrict__ tileSums, int numTiles,
-    int nb, int n, int pOut, double* __restrict__ grad, double* __restrict__ sumOut, double sumScale,
+    int nb, int n, int pOut, double* __restrict__ grad, double* __restrict__ gradHost, double* __restrict__ sumOut,
+    double* __restrict__ sumHost, double sumScale,
     PartT* __restrict__ kickX, double* __restrict__ kickMom, const double* __restrict__ kickParams) {
     __shared__ double red[kReduceThreads * (P > 1 ? P : 1)];
     const int tid = threadIdx.x;
@@ -100,7 +104,11 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
             if (tid < s) red[tid] += red[tid + s];
             __syncthreads();
         }
-        if (tid == 0) *sumOut = (0.5 * red[0]) * sumScale;  // x * 1.0 == x: the plain sum keeps its bits
+        if (tid == 0) {
+            const double v = (0.5 * red[0]) * sumScale;  // x * 1.0 == x: the plain sum keeps its bits
+            *sumOut = v;
+            if (sumHost) *sumHost = v;
+        }
         return;
     }
     const int K = blockIdx.x;
@@ -139,7 +147,10 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
             double v = 0.0;
 #pragma unroll
             for (int w = 0; w < kReduceWaves; ++w) v += red[(p * kReduceWaves + w) * kWave + lane];
-            if (row < n && p < pOut) grad[(int64_t)row * pOut + p] = v;
+            if (row < n && p < pOut) {
+                grad[(int64_t)row * pOut + p] = v;
+                if (gradHost) gradHost[(int64_t)row * pOut + p] = v;
+            }
         }
         if constexpr (KICK >= 0) kick_row<PartT, KICK>(kickX, kickMom, grad, kickParams, n, pOut, P, row);
     }
@@ -191,7 +202,8 @@ template <typename PartT, int P, int KICK>
 static void reduce_go(const ReduceLaunch& a, hipStream_t s) {
     const dim3 grid(a.gradient ? a.nb + 1 : 1);
     hipLaunchKernelGGL((reduce_kernel<PartT, P, KICK>), grid, dim3(kReduceThreads), 0, s, (const PartT*)a.part,
-                       a.tileSums, a.numTiles, a.nb, a.n, a.p, a.grad, a.sumOut, a.sumScale, (PartT*)a.kickX,
+                       a.tileSums, a.numTiles, a.nb, a.n, a.p, a.grad, a.gradHost, a.sumOut, a.sumHost, a.sumScale,
+                       (PartT*)a.kickX,
                        a.kickMom, a.kickParams);
 }
 
