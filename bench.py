@@ -565,6 +565,73 @@ def _bench_beast(steps=2000, warmup=20):
     print(json.dumps(out))
 
 
+def _bench_power(seconds=6.0, interval=0.25, n=10000):
+    """Board power and clocks under sustained load: the metric's workload
+    (N = 10,000, P = 2) evaluated back to back for `seconds` in FP64, then in
+    FP32, while a thread samples `amd-smi metric --power --clock --json`
+    every `interval` seconds (read-only queries).  Per phase: the evaluation
+    rate over the whole phase, and the median socket power and gfx clock over
+    its second half (the board's telemetry refreshes about once a second and
+    the power manager takes a second or so to settle)."""
+    import shutil
+    import subprocess
+    import threading
+
+    from massivemds_amd.engine import Flags, MultiDimensionalScaling
+
+    smi = shutil.which("amd-smi") or "/opt/rocm/bin/amd-smi"
+    samples = []
+    done = threading.Event()
+
+    def sample():
+        while not done.is_set():
+            t = time.perf_counter()
+            r = subprocess.run([smi, "metric", "--power", "--clock", "--json"], capture_output=True, text=True)
+            try:
+                g = json.loads(r.stdout)
+                g = g["gpu_data"][0] if isinstance(g, dict) else g[0]
+                clk = [v["clk"]["value"] for k, v in g["clock"].items() if k.startswith("gfx_")]
+                samples.append((t, float(g["power"]["socket_power"]["value"]), sum(clk) / len(clk)))
+            except (ValueError, KeyError, IndexError, TypeError):
+                pass
+            done.wait(interval)
+
+    _, p, tau, delta, x = _workload(n)
+    th = threading.Thread(target=sample, daemon=True)
+    th.start()
+    time.sleep(2.0)
+    phases = {}
+    for fp32 in (False, True):
+        e = MultiDimensionalScaling(p, n, Flags.OPENCL | Flags.LEFT_TRUNCATION | (Flags.FLOAT if fp32 else 0), 0)
+        e.setPairwiseData(delta)
+        e.updateLocations(-1, x)
+        e.setParameters([tau])
+        t0 = time.perf_counter()
+        count = 0
+        while time.perf_counter() - t0 < seconds:
+            for _ in range(50):
+                e.evaluate_async(True)
+            e.synchronize()
+            count += 50
+        t1 = time.perf_counter()
+        e.close()
+        phases["fp32" if fp32 else "fp64"] = (t0, t1, count / (t1 - t0))
+        time.sleep(2.0)
+    done.set()
+    th.join()
+    idle = [s for s in samples if s[0] < phases["fp64"][0]]
+    out = {"workload": f"power_n{n}_p2", "seconds_per_phase": seconds, "samples": len(samples),
+           "idle": {"socket_power_W": float(np.median([s[1] for s in idle])) if idle else None,
+                    "gfx_clock_MHz": float(np.median([s[2] for s in idle])) if idle else None}}
+    for name, (t0, t1, rate) in phases.items():
+        late = [s for s in samples if (t0 + t1) / 2 <= s[0] <= t1]
+        out[name] = {"evals_per_s": rate,
+                     "socket_power_W": float(np.median([s[1] for s in late])) if late else None,
+                     "gfx_clock_MHz": float(np.median([s[2] for s in late])) if late else None,
+                     "gfx_clock_MHz_range": [min(s[2] for s in late), max(s[2] for s in late)] if late else None}
+    return out
+
+
 def _bench_reference_loop(iterations=100, n=1000, p=2, tau=1.0, no_cpu=False, cpu_budget=10.0):
     """The reference benchmark's loop (src/benchmark.cpp:248-307) on the
     reference's own CPU-sized case: N = 1000, P = 2, FP64, euclid10
@@ -693,14 +760,15 @@ def main():
                          "for scaling studies)")
     ap.add_argument("--workload", default="n10000",
                     choices=["n10000", "n1000", "batch64", "hmc5000", "hmc5000x4", "leapfrog5000",
-                             "surrogate20000", "rowupdate10000", "beast372"],
+                             "surrogate20000", "rowupdate10000", "beast372", "power10000"],
                     help="n10000: the metric's workload (default); n1000: the reference benchmark's loop "
                          "at N = 1000 (fresh locations per iteration, through the API); batch64: the uneven batch; "
                          "hmc5000: plain HMC; hmc5000x4: four HMC chains in one launch per leapfrog step; "
                          "leapfrog5000: the HMC workload's per-step gradient pass, launched "
                          "directly (for kernel traces); surrogate20000: surrogate against plain HMC; "
                          "rowupdate10000: single-location moves through the O(N) update; "
-                         "beast372: real BEAST data (N = 372) against the logged likelihood")
+                         "beast372: real BEAST data (N = 372) against the logged likelihood; "
+                         "power10000: board power and clocks while the metric's workload runs sustained")
     ap.add_argument("--iterations", type=int, default=8000, help="sampling iterations (surrogate20000)")
     ap.add_argument("--sweep-iterations", type=int, default=2000,
                     help="sampling iterations per tuned surrogate run (surrogate20000)")
@@ -742,6 +810,9 @@ def main():
         return
     if args.workload == "beast372":
         _bench_beast()
+        return
+    if args.workload == "power10000":
+        print(json.dumps(_bench_power()))
         return
 
     n, p, tau, delta, x = _workload(args.n)
