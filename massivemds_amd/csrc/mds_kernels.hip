@@ -27,10 +27,9 @@ namespace mdship {
 // Wavefronts per workgroup of the single-instance tile kernel.  FP64 with
 // staged columns (P <= 2) runs eight, measured 2-4.5 % faster at
 // N = 5,000-20,000: the 31 KB log Phi table is copied once per eight tiles
-// instead of four.  Residency is the same either way: the symmetric P = 2
-// walk's 82 VGPRs allow 5 wavefronts per SIMD, so two eight-wavefront
-// workgroups fit per CU (4 per SIMD), the residency LDS gives four
-// four-wavefront workgroups.  Everything else keeps four.
+// instead of four.  The symmetric P = 2 walk's 76 VGPRs allow 6 wavefronts
+// per SIMD, so three eight-wavefront workgroups per CU, as many as their
+// 47 KB of LDS allow.  Everything else keeps four.
 template <typename Real, int P>
 constexpr int tile_block_waves() {
     return (sizeof(Real) == 8 && ColumnStage<Real, P>::kInLds) ? 2 * kWavesPerBlock : kWavesPerBlock;

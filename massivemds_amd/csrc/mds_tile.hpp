@@ -35,12 +35,35 @@ struct TableOf<float> {
     __device__ static float value(int i) { return coeffs::kLnPhiTab32[i]; }
 };
 
+// The value of the wavefront's first active lane, in scalar registers: for
+// values that are the same in every lane, so that the walk's arithmetic
+// takes them as scalar operands instead of reading vector registers.  The
+// empty asm statement hides where the value came from: knowing it uniform,
+// the compiler would otherwise drop the read and redo the FP arithmetic that
+// produced it, which has no scalar form, in vector registers.  The read
+// itself is the compiler's, so it keeps the wait states the hardware needs
+// between the FP64 instruction that writes the value and the read.
+__device__ __forceinline__ unsigned first_lane(unsigned v) {
+    asm volatile("" : "+v"(v));
+    return (unsigned)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ float wave_uniform(float v) {
+    return __builtin_bit_cast(float, first_lane(__builtin_bit_cast(unsigned, v)));
+}
+__device__ __forceinline__ double wave_uniform(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = first_lane((unsigned)b);
+    const unsigned hi = first_lane((unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Per-evaluation constants from the precision (as the host holds it, or
-// read from a sampler's device parameter block), computed in FP64.
+// read from a sampler's device parameter block), computed in FP64; the same
+// in every lane, so held in scalar registers.
 template <typename Real>
 __device__ __forceinline__ PairConst<Real> make_pair_const(double tau, const Real* tab) {
     const double sT = TableOf<Real>::kStep * sqrt(tau);
-    return {(Real)(0.5 * tau), (Real)sT, (Real)(0.5 * sT), tab};
+    return {wave_uniform((Real)(0.5 * tau)), wave_uniform((Real)sT), wave_uniform((Real)(0.5 * sT)), tab};
 }
 
 // Copies the log Phi table of this precision into LDS with every thread of
