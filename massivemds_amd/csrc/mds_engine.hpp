@@ -77,7 +77,7 @@ struct Engine {
     double* dTileSums = nullptr;
     void* dPart = nullptr;     // gradient partials in the engine's real type
     double* dGrad = nullptr;
-    double* dSum = nullptr;    // dGrad + N * P: the sum follows the gradient, so one copy reads both
+    double* dSum = nullptr;    // dGrad + N * P: the sum follows the gradient, as in hGrad
     double* hGrad = nullptr;   // pinned, coherent (the reduce kernel can write it directly)
     double* kGrad = nullptr;   // its device address
     double* hSum = nullptr;    // pinned, coherent: the reduce kernel can write it directly
