@@ -897,6 +897,9 @@ def main():
             # iterations per chain and the complete sweep, is --workload surrogate20000)
             "surrogate_n20000_p5_short": _bench_surrogate(iterations=500, sweep_iterations=500, sweep=(2, 10),
                                                           grid=False),
+            # the metric's workload sustained for 4 s per precision, with the
+            # board's power and clocks (--workload power10000 runs 6 s)
+            "power_n10000": _bench_power(seconds=4.0),
         }
     print(json.dumps(out))
 
