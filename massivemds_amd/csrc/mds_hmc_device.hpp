@@ -70,8 +70,44 @@ __device__ __forceinline__ void kick_row(Real* __restrict__ x, double* __restric
 }
 
 // out[0] = sum mom^2 / 2, out[1] = sum_i x_i^T T x_i (fixed-order
-// reductions over one block of kEnergyThreads threads).
+// reductions over one block of kEnergyThreads threads: thread t adds rows t,
+// t + kEnergyThreads, ... in turn, then a fixed tree).
 constexpr int kEnergyThreads = 512;
+
+// Thread t's rows for P coordinates: the loads of kAhead rows are issued
+// before their additions, which keep the row order and each row's
+// expressions (the same bits as adding row after row; the block is
+// otherwise bound by one load's latency per row).
+template <typename Real, int P>
+__device__ __forceinline__ void energy_rows(const Real* __restrict__ x, const double* __restrict__ mom,
+                                            const double* __restrict__ T, int n, int kdim, double& k, double& q) {
+    constexpr int kAhead = P <= 2 ? 8 : (P <= 4 ? 4 : (P <= 8 ? 2 : 1));
+    for (int i0 = threadIdx.x; i0 < n; i0 += kAhead * kEnergyThreads) {
+        double m[kAhead][P], xv[kAhead][P];
+#pragma unroll
+        for (int a = 0; a < kAhead; ++a) {
+            const int i = i0 + a * kEnergyThreads;
+#pragma unroll
+            for (int d = 0; d < P; ++d) {
+                m[a][d] = i < n ? mom[(int64_t)i * P + d] : 0.0;
+                xv[a][d] = i < n ? (double)x[(int64_t)i * kdim + d] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < kAhead; ++a) {
+            if (i0 + a * kEnergyThreads >= n) break;
+#pragma unroll
+            for (int d = 0; d < P; ++d) {
+                const double md = m[a][d];
+                k += 0.5 * md * md;
+                double row = 0.0;
+#pragma unroll
+                for (int e = 0; e < P; ++e) row += xv[a][e] * T[e * P + d];
+                q += row * xv[a][d];
+            }
+        }
+    }
+}
 
 template <typename Real>
 __device__ __forceinline__ void energy_block(const Real* __restrict__ x, const double* __restrict__ mom,
@@ -79,15 +115,24 @@ __device__ __forceinline__ void energy_block(const Real* __restrict__ x, const d
                                              double* __restrict__ out, double (*red)[kEnergyThreads]) {
     const double* T = params + kHmcTrait;
     double k = 0.0, q = 0.0;
-    for (int i = threadIdx.x; i < n; i += kEnergyThreads) {
-        const Real* xi = x + (int64_t)i * kdim;
-        for (int d = 0; d < p; ++d) {
-            const double m = mom[(int64_t)i * p + d];
-            k += 0.5 * m * m;
-            double row = 0.0;
-            for (int e = 0; e < p; ++e) row += (double)xi[e] * T[e * p + d];
-            q += row * (double)xi[d];
-        }
+    switch (p) {
+        case 1: energy_rows<Real, 1>(x, mom, T, n, kdim, k, q); break;
+        case 2: energy_rows<Real, 2>(x, mom, T, n, kdim, k, q); break;
+        case 3: energy_rows<Real, 3>(x, mom, T, n, kdim, k, q); break;
+        case 4: energy_rows<Real, 4>(x, mom, T, n, kdim, k, q); break;
+        case 5: energy_rows<Real, 5>(x, mom, T, n, kdim, k, q); break;
+        case 6: energy_rows<Real, 6>(x, mom, T, n, kdim, k, q); break;
+        case 7: energy_rows<Real, 7>(x, mom, T, n, kdim, k, q); break;
+        case 8: energy_rows<Real, 8>(x, mom, T, n, kdim, k, q); break;
+        case 9: energy_rows<Real, 9>(x, mom, T, n, kdim, k, q); break;
+        case 10: energy_rows<Real, 10>(x, mom, T, n, kdim, k, q); break;
+        case 11: energy_rows<Real, 11>(x, mom, T, n, kdim, k, q); break;
+        case 12: energy_rows<Real, 12>(x, mom, T, n, kdim, k, q); break;
+        case 13: energy_rows<Real, 13>(x, mom, T, n, kdim, k, q); break;
+        case 14: energy_rows<Real, 14>(x, mom, T, n, kdim, k, q); break;
+        case 15: energy_rows<Real, 15>(x, mom, T, n, kdim, k, q); break;
+        case 16: energy_rows<Real, 16>(x, mom, T, n, kdim, k, q); break;
+        default: break;  // the samplers admit 1 <= P <= kMaxDim = 16
     }
     red[0][threadIdx.x] = k;
     red[1][threadIdx.x] = q;
