@@ -167,9 +167,24 @@ constexpr int kGramThreads = 256;
 template <typename Real>
 __device__ __forceinline__ void gram_entry(const Real* __restrict__ x, int n, int kdim, int p, int a, int b,
                                            double* __restrict__ out, double* red) {
+    // the loads of kAhead rows ahead of their additions, which keep the
+    // thread's row order (the same bits as one row after another)
+    constexpr int kAhead = 8;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kGramThreads)
-        acc += (double)x[(int64_t)i * kdim + a] * (double)x[(int64_t)i * kdim + b];
+    for (int i0 = threadIdx.x; i0 < n; i0 += kAhead * kGramThreads) {
+        double va[kAhead], vb[kAhead];
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            const int i = i0 + u * kGramThreads;
+            va[u] = i < n ? (double)x[(int64_t)i * kdim + a] : 0.0;
+            vb[u] = i < n ? (double)x[(int64_t)i * kdim + b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            if (i0 + u * kGramThreads >= n) break;
+            acc += va[u] * vb[u];
+        }
+    }
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int s = kGramThreads / 2; s > 0; s >>= 1) {
