@@ -275,9 +275,8 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
     auto load_group = [&](Real (&b)[G], Real (&bT)[G], int k0) {
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            const int kn = min(k0 + g, kLast);  // past the walk: re-read its last step
-            b[g] = tileStep[kn * kTile];
-            bT[g] = SYM ? b[g] : tileStep[planeStride + kn * kTile];
+            b[g] = tileStep[(k0 + g) * kTile];
+            bT[g] = SYM ? b[g] : tileStep[planeStride + (k0 + g) * kTile];
         }
     };
 
@@ -336,7 +335,9 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
         load_group(bufB, bufBT, k0 + G);
         __builtin_amdgcn_sched_barrier(0);
         run_group(bufA, bufAT, k0);
-        load_group(bufA, bufAT, k0 + 2 * G);
+        // the walk's last pass re-reads its last group instead of reading
+        // past it (one clamp for the group: its loads share one address)
+        load_group(bufA, bufAT, min(k0 + 2 * G, kLast + 1 - G));
         __builtin_amdgcn_sched_barrier(0);
         run_group(bufB, bufBT, k0 + G);
         // every fourth pass (8 G steps) the FP32 group sums enter their FP64
