@@ -35,19 +35,25 @@ constexpr int tile_block_waves() {
     return (sizeof(Real) == 8 && ColumnStage<Real, P>::kInLds) ? 2 * kWavesPerBlock : kWavesPerBlock;
 }
 
+// Small instances, at most kSmallTiles tiles (N <= 2,048), run the
+// eight-wavefront kernels as four-wavefront workgroups: their few tiles then
+// spread over twice as many CUs, one wavefront per SIMD, and a pass lasts
+// about one tile walk (measured, FP64 P = 2: 19.1 -> 16.5 us per tile pass
+// at N = 500-1,500, even at N = 2,000, 5-8 % slower from N = 3,000 up).
+// Each tile's walk is the same in either form, so are the results.
+constexpr int kSmallTiles = 528;
+
 // Single instance: one wavefront per tile of the upper block triangle.  The
 // truncated forms copy the log Phi table into LDS and every wavefront stages
-// its tile's column locations (small P); all wavefronts of the block take
-// part in the barrier before any leaves.
-template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
-__global__ __launch_bounds__((tile_block_waves<Real, P>() * kWave)) void tile_kernel(
-    const Real* __restrict__ delta, const Real* __restrict__ x,
-    const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
-    double tau, double* __restrict__ tileSums, Real* __restrict__ part, const double* __restrict__ devTau) {
-    constexpr int kWaves = tile_block_waves<Real, P>();
-    __shared__ __align__(16) Real lnPhiTab[TRUNC ? TableOf<Real>::kCount : 1];
-    __shared__ __align__(16) Real colStage[kWaves][ColumnStage<Real, P>::kElems];
-    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + threadIdx.x / kWave);
+// its tile's column locations (small P); all WAVES wavefronts of the block
+// take part in the barrier before any leaves.
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, int WAVES>
+__device__ __forceinline__ void tile_block(Real* __restrict__ lnPhiTab, Real (*colStage)[ColumnStage<Real, P>::kElems],
+                                           const Real* __restrict__ delta, const Real* __restrict__ x,
+                                           const uint32_t* __restrict__ tiles, int numTiles, int nb,
+                                           int64_t planeStride, double tau, double* __restrict__ tileSums,
+                                           Real* __restrict__ part, const double* __restrict__ devTau) {
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + threadIdx.x / kWave);
     const uint32_t code = t < numTiles ? tiles[t] : 0u;
     Real* stage = colStage[threadIdx.x / kWave];
     if (t < numTiles) stage_columns<Real, P>(stage, x, (int)(code & 0xffffu));
@@ -58,6 +64,30 @@ __global__ __launch_bounds__((tile_block_waves<Real, P>() * kWave)) void tile_ke
     tile_walk<Real, P, TRUNC, SYM, GRAD, kGroup>(delta + (int64_t)t * kTileElems, planeStride, x, stage,
                                                  (int)(code >> 16), (int)(code & 0xffffu), nb, pc, tileSums + t,
                                                  part);
+}
+
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
+__global__ __launch_bounds__((tile_block_waves<Real, P>() * kWave)) void tile_kernel(
+    const Real* __restrict__ delta, const Real* __restrict__ x,
+    const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
+    double tau, double* __restrict__ tileSums, Real* __restrict__ part, const double* __restrict__ devTau) {
+    constexpr int kWaves = tile_block_waves<Real, P>();
+    __shared__ __align__(16) Real lnPhiTab[TRUNC ? TableOf<Real>::kCount : 1];
+    __shared__ __align__(16) Real colStage[kWaves][ColumnStage<Real, P>::kElems];
+    tile_block<Real, P, TRUNC, SYM, GRAD, kWaves>(lnPhiTab, colStage, delta, x, tiles, numTiles, nb, planeStride,
+                                                  tau, tileSums, part, devTau);
+}
+
+// The four-wavefront form of the eight-wavefront kernels, for small instances.
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
+__global__ __launch_bounds__(kWavesPerBlock * kWave) void tile_kernel_small(
+    const Real* __restrict__ delta, const Real* __restrict__ x,
+    const uint32_t* __restrict__ tiles, int numTiles, int nb, int64_t planeStride,
+    double tau, double* __restrict__ tileSums, Real* __restrict__ part, const double* __restrict__ devTau) {
+    __shared__ __align__(16) Real lnPhiTab[TRUNC ? TableOf<Real>::kCount : 1];
+    __shared__ __align__(16) Real colStage[kWavesPerBlock][ColumnStage<Real, P>::kElems];
+    tile_block<Real, P, TRUNC, SYM, GRAD, kWavesPerBlock>(lnPhiTab, colStage, delta, x, tiles, numTiles, nb,
+                                                          planeStride, tau, tileSums, part, devTau);
 }
 
 constexpr int kReduceWaves = 8;
@@ -158,6 +188,15 @@ __global__ __launch_bounds__(kReduceThreads) void reduce_kernel(
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
 static hipError_t go(const TileLaunch& a, hipStream_t s) {
     constexpr int kWaves = tile_block_waves<Real, P>();
+    if constexpr (kWaves != kWavesPerBlock) {
+        if (a.numTiles <= kSmallTiles) {
+            const dim3 grid((a.numTiles + kWavesPerBlock - 1) / kWavesPerBlock);
+            hipLaunchKernelGGL((tile_kernel_small<Real, P, TRUNC, SYM, GRAD>), grid, dim3(kWavesPerBlock * kWave), 0,
+                               s, (const Real*)a.delta, (const Real*)a.x, a.tiles, a.numTiles, a.nb, a.planeStride,
+                               a.tau, a.tileSums, (Real*)a.part, a.devTau);
+            return hipGetLastError();
+        }
+    }
     const dim3 grid((a.numTiles + kWaves - 1) / kWaves);
     hipLaunchKernelGGL((tile_kernel<Real, P, TRUNC, SYM, GRAD>), grid, dim3(kWaves * kWave), 0, s,
                        (const Real*)a.delta, (const Real*)a.x, a.tiles, a.numTiles, a.nb, a.planeStride, a.tau,
