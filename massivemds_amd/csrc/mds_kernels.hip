@@ -40,14 +40,16 @@ constexpr int tile_block_waves() {
 // spread over twice as many CUs, one wavefront per SIMD, and a pass lasts
 // about one tile walk (measured, FP64 P = 2: 19.1 -> 16.5 us per tile pass
 // at N = 500-1,500, even at N = 2,000, 5-8 % slower from N = 3,000 up).
-// Each tile's walk is the same in either form, so are the results.
+// Their FP64 walks compute each group's terms ahead of its additions (3 %
+// faster there, one wavefront per SIMD; slower in fuller launches).  Each
+// tile's additions keep their order in either form, so do the results.
 constexpr int kSmallTiles = 528;
 
 // Single instance: one wavefront per tile of the upper block triangle.  The
 // truncated forms copy the log Phi table into LDS and every wavefront stages
 // its tile's column locations (small P); all WAVES wavefronts of the block
-// take part in the barrier before any leaves.
-template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, int WAVES>
+// take part in the barrier before any leaves; TERMS_FIRST as in tile_walk.
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, int WAVES, bool TERMS_FIRST>
 __device__ __forceinline__ void tile_block(Real* __restrict__ lnPhiTab, Real (*colStage)[ColumnStage<Real, P>::kElems],
                                            const Real* __restrict__ delta, const Real* __restrict__ x,
                                            const uint32_t* __restrict__ tiles, int numTiles, int nb,
@@ -61,9 +63,9 @@ __device__ __forceinline__ void tile_block(Real* __restrict__ lnPhiTab, Real (*c
     __syncthreads();
     if (t >= numTiles) return;
     const PairConst<Real> pc = make_pair_const<Real>(devTau ? *devTau : tau, lnPhiTab);
-    tile_walk<Real, P, TRUNC, SYM, GRAD, kGroup>(delta + (int64_t)t * kTileElems, planeStride, x, stage,
-                                                 (int)(code >> 16), (int)(code & 0xffffu), nb, pc, tileSums + t,
-                                                 part);
+    tile_walk<Real, P, TRUNC, SYM, GRAD, kGroup, TERMS_FIRST>(delta + (int64_t)t * kTileElems, planeStride, x,
+                                                              stage, (int)(code >> 16), (int)(code & 0xffffu), nb,
+                                                              pc, tileSums + t, part);
 }
 
 template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD>
@@ -74,8 +76,9 @@ __global__ __launch_bounds__((tile_block_waves<Real, P>() * kWave)) void tile_ke
     constexpr int kWaves = tile_block_waves<Real, P>();
     __shared__ __align__(16) Real lnPhiTab[TRUNC ? TableOf<Real>::kCount : 1];
     __shared__ __align__(16) Real colStage[kWaves][ColumnStage<Real, P>::kElems];
-    tile_block<Real, P, TRUNC, SYM, GRAD, kWaves>(lnPhiTab, colStage, delta, x, tiles, numTiles, nb, planeStride,
-                                                  tau, tileSums, part, devTau);
+    tile_block<Real, P, TRUNC, SYM, GRAD, kWaves, (sizeof(Real) == 4)>(lnPhiTab, colStage, delta, x, tiles,
+                                                                       numTiles, nb, planeStride, tau, tileSums,
+                                                                       part, devTau);
 }
 
 // The four-wavefront form of the eight-wavefront kernels, for small instances.
@@ -86,8 +89,8 @@ __global__ __launch_bounds__(kWavesPerBlock * kWave) void tile_kernel_small(
     double tau, double* __restrict__ tileSums, Real* __restrict__ part, const double* __restrict__ devTau) {
     __shared__ __align__(16) Real lnPhiTab[TRUNC ? TableOf<Real>::kCount : 1];
     __shared__ __align__(16) Real colStage[kWavesPerBlock][ColumnStage<Real, P>::kElems];
-    tile_block<Real, P, TRUNC, SYM, GRAD, kWavesPerBlock>(lnPhiTab, colStage, delta, x, tiles, numTiles, nb,
-                                                          planeStride, tau, tileSums, part, devTau);
+    tile_block<Real, P, TRUNC, SYM, GRAD, kWavesPerBlock, true>(lnPhiTab, colStage, delta, x, tiles, numTiles,
+                                                                nb, planeStride, tau, tileSums, part, devTau);
 }
 
 constexpr int kReduceWaves = 8;

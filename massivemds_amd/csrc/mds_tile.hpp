@@ -252,7 +252,7 @@ __device__ __forceinline__ void pair_accumulate(TileAcc<Real, P>& acc, const Pai
 // 64 x 64 tile's terms for 64 rows, which the reduce adds in FP64; stored in
 // FP32 by the FP32 engine, whose per-term arithmetic already carries FP32
 // rounding, so that its partial traffic halves).
-template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, bool DIAG, int G>
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, bool DIAG, int G, bool TERMS_FIRST>
 __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int64_t planeStride,
                                           const Real* __restrict__ x, const Real* __restrict__ stage, int I,
                                           int J, int nb, const PairConst<Real>& pc, double* __restrict__ tileSum,
@@ -301,14 +301,18 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
 #pragma unroll
                 for (int p = 0; p < P; ++p) xs[g][p] = stageGroup[CS::at(g, p)];
         }
-        // FP32: the group's terms first (one block, in which the steps' work
-        // interleaves), then their additions step by step; FP64: each step's
-        // additions right after its terms (the compiler then moves the whole
-        // step under the observation mask).  Measured on the tile kernel at
-        // N = 10,000 against selects of zeros: FP32 -4 % (P = 2) to -7.5 %
-        // (P = 10) this way, +5 % the other; FP64 -2 % this way, +5 % the
-        // other (its terms of four steps at once contend for the LDS table).
-        constexpr bool kGroupTermsFirst = sizeof(Real) == 4;
+        // TERMS_FIRST: the group's terms first (one block, in which the
+        // steps' work interleaves), then their additions step by step;
+        // otherwise each step's additions right after its terms (the
+        // compiler then moves the whole step under the observation mask).
+        // The additions keep their order either way, so do the results.
+        // Measured on the tile kernel at N = 10,000 against selects of
+        // zeros: FP32 -4 % (P = 2) to -7.5 % (P = 10) with the terms first,
+        // +5 % the other way; FP64 -2 % step by step, +5 % the other way (its
+        // terms of four steps at once contend for the LDS table), but -3 %
+        // with the terms first where one wavefront per SIMD walks (small
+        // instances, whose walks are latency-bound).
+        constexpr bool kGroupTermsFirst = TERMS_FIRST;
         PairTerms<Real, P> terms[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -377,16 +381,20 @@ __device__ __forceinline__ void tile_body(const Real* __restrict__ tileBase, int
 }
 
 // Both walks of the tile (I, J): diagonal tiles take the half walk.
-template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, int G>
+// TERMS_FIRST: the schedule of a group's steps (see tile_body); FP32 walks
+// take the terms first, FP64 walks step by step except in the
+// small-instance kernels.
+template <typename Real, int P, bool TRUNC, bool SYM, bool GRAD, int G, bool TERMS_FIRST = (sizeof(Real) == 4)>
 __device__ __forceinline__ void tile_walk(const Real* __restrict__ tileBase, int64_t planeStride,
                                           const Real* __restrict__ x, const Real* __restrict__ stage, int I,
                                           int J, int nb, const PairConst<Real>& pc, double* __restrict__ tileSum,
                                           Real* __restrict__ part) {
     if (I == J)
-        tile_body<Real, P, TRUNC, SYM, GRAD, true, G>(tileBase, planeStride, x, stage, I, J, nb, pc, tileSum, part);
+        tile_body<Real, P, TRUNC, SYM, GRAD, true, G, TERMS_FIRST>(tileBase, planeStride, x, stage, I, J, nb, pc,
+                                                                   tileSum, part);
     else
-        tile_body<Real, P, TRUNC, SYM, GRAD, false, G>(tileBase, planeStride, x, stage, I, J, nb, pc, tileSum,
-                                                        part);
+        tile_body<Real, P, TRUNC, SYM, GRAD, false, G, TERMS_FIRST>(tileBase, planeStride, x, stage, I, J, nb, pc,
+                                                                    tileSum, part);
 }
 
 }  // namespace mdship
