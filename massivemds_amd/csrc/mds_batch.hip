@@ -70,7 +70,11 @@ struct ReduceItem {
 
 constexpr int kBatchMaxThreads = 512;
 
-template <typename Real, int P>
+// FLAGS >= 0: every item of the launch has truncation = FLAGS & 1 and
+// symmetric observations = FLAGS & 2 (one walk compiled, fewer registers);
+// FLAGS = -1: the walk is chosen per item.  The walks are the same either
+// way, so are the results.
+template <typename Real, int P, int FLAGS = -1>
 __device__ __forceinline__ void batch_tile(const BatchDataset& d, uint32_t t, double* __restrict__ tileSums,
                                            const Real* lnPhiTab, Real* stage) {
     const uint32_t code = d.tiles[t];
@@ -84,7 +88,10 @@ __device__ __forceinline__ void batch_tile(const BatchDataset& d, uint32_t t, do
     wave_lds_sync();
     stage_columns<Real, P>(stage, x, J);
     wave_lds_sync();
-    if (d.truncated) {
+    if constexpr (FLAGS >= 0) {
+        tile_walk<Real, P, (FLAGS & 1) != 0, (FLAGS & 2) != 0, true, kGroup>(base, d.planeStride, x, stage, I, J,
+                                                                             d.nb, pc, sum, part);
+    } else if (d.truncated) {
         if (d.symmetric)
             tile_walk<Real, P, true, true, true, kGroup>(base, d.planeStride, x, stage, I, J, d.nb, pc, sum, part);
         else
@@ -124,7 +131,7 @@ constexpr int batch_stage_elems() {
     return KDIM == 0 ? ColumnStage<Real, 1>::kMaxElems : ColumnStage<Real, KDIM>::kElems;
 }
 
-template <typename Real, int KDIM>
+template <typename Real, int KDIM, int FLAGS = -1>
 __global__ __launch_bounds__(kBatchMaxThreads) void batch_tile_kernel(const BatchDataset* __restrict__ ds,
                                                                        const BatchItem* __restrict__ items,
                                                                        int numItems,
@@ -145,7 +152,7 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batch_tile_kernel(const Batc
             if constexpr (KDIM == 0)
                 batch_tile_any<Real>(d, it.tile + c, tileSums, lnPhiTab, stage);
             else
-                batch_tile<Real, KDIM>(d, it.tile + c, tileSums, lnPhiTab, stage);
+                batch_tile<Real, KDIM, FLAGS>(d, it.tile + c, tileSums, lnPhiTab, stage);
         }
     }
 }
@@ -303,9 +310,10 @@ struct Batch {
     ReduceItem* dReduce = nullptr;
     int numReduce = 0;
     // launch plan: (kdim class or 0, first item, item count, first reduce
-    // item, reduce item count); each launch's reduce follows it on its stream
+    // item, reduce item count, the flags its datasets share or -1); each
+    // launch's reduce follows it on its stream
     struct Launch {
-        int kdim, first, count, redFirst, redCount;
+        int kdim, first, count, redFirst, redCount, flags;
     };
     std::vector<Launch> plan;
     BatchConfig cfg;
@@ -568,9 +576,15 @@ struct Batch {
                 red.push_back({k, -1});
                 for (int K = 0; K < e[k].nb; ++K) red.push_back({k, K});
             }
-            plan.push_back({cls.first, first, (int)items.size() - first, redFirst, (int)red.size() - redFirst});
+            // the walk variant every dataset of the class takes, if they agree
+            int flags = -2;
+            for (int k : cls.second) {
+                const int f = desc[k].truncated | (desc[k].symmetric << 1);
+                flags = flags == -2 || flags == f ? f : -1;
+            }
+            plan.push_back({cls.first, first, (int)items.size() - first, redFirst, (int)red.size() - redFirst, flags});
         }
-        if (cfg.strategy == 0) plan = {{0, 0, (int)items.size(), 0, (int)red.size()}};
+        if (cfg.strategy == 0) plan = {{0, 0, (int)items.size(), 0, (int)red.size(), -1}};
         numReduce = (int)red.size();
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         (void)hipFree(dItems);
@@ -585,28 +599,43 @@ struct Batch {
         planDirty = false;
     }
 
-    template <typename Real, int KDIM>
-    void launchTiles(const BatchItem* items, int count, hipStream_t stream) {
+    template <typename Real, int KDIM, int FLAGS>
+    void launchTilesAs(const BatchItem* items, int count, hipStream_t stream) {
         const int wpb = cfg.wavesPerBlock;
         const int want = (count + wpb - 1) / wpb;
         const int grid = std::max(1, std::min(want, cuCount * cfg.blocksPerCU));
         const size_t stageBytes = (size_t)wpb * batch_stage_elems<Real, KDIM>() * sizeof(Real);
-        hipLaunchKernelGGL((batch_tile_kernel<Real, KDIM>), dim3(grid), dim3(wpb * kWave), stageBytes, stream, dDesc,
-                           items, count, dTileSums);
+        hipLaunchKernelGGL((batch_tile_kernel<Real, KDIM, FLAGS>), dim3(grid), dim3(wpb * kWave), stageBytes, stream,
+                           dDesc, items, count, dTileSums);
+    }
+
+    // flags: the walk variant every item of the launch takes, or -1
+    template <typename Real, int KDIM>
+    void launchTiles(const BatchItem* items, int count, hipStream_t stream, int flags) {
+        if constexpr (KDIM > 0) {
+            switch (flags) {
+                case 0: launchTilesAs<Real, KDIM, 0>(items, count, stream); return;
+                case 1: launchTilesAs<Real, KDIM, 1>(items, count, stream); return;
+                case 2: launchTilesAs<Real, KDIM, 2>(items, count, stream); return;
+                case 3: launchTilesAs<Real, KDIM, 3>(items, count, stream); return;
+                default: break;
+            }
+        }
+        launchTilesAs<Real, KDIM, -1>(items, count, stream);
     }
 
     template <typename Real>
-    void launchClass(int kdim, const BatchItem* items, int count, hipStream_t stream) {
+    void launchClass(int kdim, const BatchItem* items, int count, hipStream_t stream, int flags) {
         switch (kdim) {
-            case 0: launchTiles<Real, 0>(items, count, stream); break;
-            case 1: launchTiles<Real, 1>(items, count, stream); break;
-            case 2: launchTiles<Real, 2>(items, count, stream); break;
-            case 3: launchTiles<Real, 3>(items, count, stream); break;
-            case 4: launchTiles<Real, 4>(items, count, stream); break;
-            case 5: launchTiles<Real, 5>(items, count, stream); break;
-            case 6: launchTiles<Real, 6>(items, count, stream); break;
-            case 8: launchTiles<Real, 8>(items, count, stream); break;
-            case 10: launchTiles<Real, 10>(items, count, stream); break;
+            case 0: launchTiles<Real, 0>(items, count, stream, flags); break;
+            case 1: launchTiles<Real, 1>(items, count, stream, flags); break;
+            case 2: launchTiles<Real, 2>(items, count, stream, flags); break;
+            case 3: launchTiles<Real, 3>(items, count, stream, flags); break;
+            case 4: launchTiles<Real, 4>(items, count, stream, flags); break;
+            case 5: launchTiles<Real, 5>(items, count, stream, flags); break;
+            case 6: launchTiles<Real, 6>(items, count, stream, flags); break;
+            case 8: launchTiles<Real, 8>(items, count, stream, flags); break;
+            case 10: launchTiles<Real, 10>(items, count, stream, flags); break;
             default: throw Error(MDS_HIP_ERR_ARGUMENT, "unsupported kernel dimension");
         }
         check(hipGetLastError(), "batch tile kernel");
@@ -652,9 +681,9 @@ struct Batch {
                 }
             }
             if (fp32)
-                launchClass<float>(l.kdim, dItems + l.first, l.count, s);
+                launchClass<float>(l.kdim, dItems + l.first, l.count, s, l.flags);
             else
-                launchClass<double>(l.kdim, dItems + l.first, l.count, s);
+                launchClass<double>(l.kdim, dItems + l.first, l.count, s, l.flags);
             // the class's datasets reduce while other classes still walk
             if (l.redCount > 0) launchReduce(dReduce + l.redFirst, l.redCount, s, kickForm);
         }
