@@ -139,6 +139,41 @@ def test_schedule_independent_bitwise(tmp_path):
     b.close()
 
 
+@pytest.mark.parametrize("truncated", [True, False])
+def test_uniform_asymmetric_class(oracle, truncated):
+    """A class launch whose datasets all have asymmetric observations runs the
+    walk compiled for that variant alone (mds_batch.hip: batch_tile_kernel
+    with FLAGS >= 0), the one-launch strategy the walk chosen per item: both
+    must equal the single-instance engine bit for bit and the FP64 oracle
+    to 1e-12 (src/NewMultiDimensionalScaling.hpp:421-461 for the rows'
+    orientation)."""
+    flags = Flags.OPENCL | (Flags.LEFT_TRUNCATION if truncated else 0)
+    specs = [(150, 2, 1.3, 11), (90, 2, 0.7, 12), (120, 3, 2.1, 13)]
+    data = []
+    for n, p, tau, seed in specs:
+        delta, x = datagen.batch_dataset(n, p, seed)
+        if p == 2:
+            delta = np.abs(delta + 0.05 * datagen.normals(seed, 9, n * n).reshape(n, n))
+            np.fill_diagonal(delta, 0.0)
+        data.append((delta, x, tau))
+    singles = [_single(delta, x, tau, flags) for delta, x, tau in data]
+    b = _batch(specs, data, flags)
+    for strategy in (1, 0):
+        print(f"uniform asymmetric class truncated={truncated} strategy={strategy}", flush=True)
+        b.set_config(4, 2, 1, strategy)
+        b.evaluate()
+        sums = b.sums()
+        for k, (delta, x, tau) in enumerate(data):
+            s, g = singles[k]
+            assert sums[k] == s, (strategy, k, sums[k], s)
+            np.testing.assert_array_equal(b.gradient(k), g)
+            s_ref = oracle.sum_of_increments(delta, x, tau, truncated)
+            g_ref = oracle.gradient(delta, x, tau, truncated)
+            assert abs(s - s_ref) <= 1e-12 * abs(s_ref), (k, s, s_ref)
+            assert np.max(np.abs(g - g_ref)) <= 1e-12 * np.max(np.abs(g_ref)), k
+    b.close()
+
+
 def test_batch_errors():
     with pytest.raises(MdsError):
         MultiDimensionalScalingBatch([2, 11], [50, 60], Flags.OPENCL)
