@@ -284,6 +284,22 @@ def test_reference_benchmark_loop(oracle):
         assert np.max(np.abs(g - g_ref)) <= RTOL64 * np.max(np.abs(g_ref))
 
 
+@pytest.mark.parametrize("p,pattern", [(2, "asym"), (2, "sym5"), (1, "row")])
+@pytest.mark.parametrize("truncated", [True, False])
+def test_eight_wavefront_launch(oracle, p, pattern, truncated):
+    """FP64 walks with P <= 2 run as four-wavefront workgroups up to 528
+    tiles (N <= 2,048) and as eight-wavefront ones above: every variant of
+    the larger launch (truncated or not, symmetric or asymmetric
+    observations, a missing row) against the oracle at N = 2,200."""
+    n, tau = 2200, 1.7
+    print(f"eight-wavefront launch n={n} p={p} pattern={pattern} truncated={truncated}", flush=True)
+    delta, x = _data(n, p, pattern)
+    e, s, g = _gpu(delta, x, tau, truncated)
+    e.close()
+    _check(s, g, oracle.sum_of_increments(delta, x, tau, truncated), oracle.gradient(delta, x, tau, truncated),
+           RTOL64)
+
+
 def test_full_size_n10000(oracle):
     """N = 10,000, P = 2, tau = 2, 5% missing, noise 0.1: FP64 against the
     multi-core oracle (validated against the serial one at small N), FP32
