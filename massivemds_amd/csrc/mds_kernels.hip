@@ -35,15 +35,17 @@ constexpr int tile_block_waves() {
     return (sizeof(Real) == 8 && ColumnStage<Real, P>::kInLds) ? 2 * kWavesPerBlock : kWavesPerBlock;
 }
 
-// Small instances, at most kSmallTiles tiles (N <= 2,048), run the
+// Small instances, at most kSmallTiles tiles (N <= 2,816: at most one
+// four-wavefront workgroup per CU of the MI355X's 256), run the
 // eight-wavefront kernels as four-wavefront workgroups: their few tiles then
 // spread over twice as many CUs, one wavefront per SIMD, and a pass lasts
-// about one tile walk (measured, FP64 P = 2: 19.1 -> 16.5 us per tile pass
-// at N = 500-1,500, even at N = 2,000, 5-8 % slower from N = 3,000 up).
+// about one tile walk (measured, FP64 P = 2: 19.1 -> 16.0 us per tile pass
+// at N = 500-1,500, 4-10 % faster at N = 2,100-2,700, 8 % slower at
+// N = 2,900, 5-8 % slower from N = 3,000 up).
 // Their FP64 walks compute each group's terms ahead of its additions (3 %
 // faster there, one wavefront per SIMD; slower in fuller launches).  Each
 // tile's additions keep their order in either form, so do the results.
-constexpr int kSmallTiles = 528;
+constexpr int kSmallTiles = 1024;
 
 // Single instance: one wavefront per tile of the upper block triangle.  The
 // truncated forms copy the log Phi table into LDS and every wavefront stages

@@ -287,11 +287,11 @@ def test_reference_benchmark_loop(oracle):
 @pytest.mark.parametrize("p,pattern", [(2, "asym"), (2, "sym5"), (1, "row")])
 @pytest.mark.parametrize("truncated", [True, False])
 def test_eight_wavefront_launch(oracle, p, pattern, truncated):
-    """FP64 walks with P <= 2 run as four-wavefront workgroups up to 528
-    tiles (N <= 2,048) and as eight-wavefront ones above: every variant of
+    """FP64 walks with P <= 2 run as four-wavefront workgroups up to 1024
+    tiles (N <= 2,816) and as eight-wavefront ones above: every variant of
     the larger launch (truncated or not, symmetric or asymmetric
-    observations, a missing row) against the oracle at N = 2,200."""
-    n, tau = 2200, 1.7
+    observations, a missing row) against the oracle at N = 3,000."""
+    n, tau = 3000, 1.7
     print(f"eight-wavefront launch n={n} p={p} pattern={pattern} truncated={truncated}", flush=True)
     delta, x = _data(n, p, pattern)
     e, s, g = _gpu(delta, x, tau, truncated)
